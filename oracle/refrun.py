@@ -1,0 +1,68 @@
+"""Run the reference's own Fortran hot path through oracle/_ref/refharness.
+
+TEST INFRASTRUCTURE ONLY.  oracle/ref/Makefile builds the harness from the
+reference sources under /root/reference/f (FP64 via -fdefault-real-8, and the
+FP32 build as shipped).  Used by tests/golden/make_golden.py to write golden
+vectors and by CPU tests that check the C restatement when the harness exists.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(_HERE, "_ref")
+
+CMD = dict(geom=1, staples=2, agauge=3, a_force=4, ahiggs1=5, ahiggs2=6,
+           ahiggs3=7, renorm=8, or_gauge_fixing=9, landau_gauge=10, sweep=11,
+           heat_kpi=12, gauss=13)
+
+
+def harness_path(prec=64):
+    return os.path.join(REF_DIR, "refharness%d" % prec)
+
+
+def available(prec=64):
+    return os.access(harness_path(prec), os.X_OK)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "ref")],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
+        beta=9.0, beta_a=0.0, beta_2=1.0, beta_4=0.0, lam=1.0, alpha=1.0,
+        limit_or=1e-9, delta=0.4, ex1=None, ex2=None, prec=64):
+    """Run one harness command; returns the list of output records."""
+    lx, ly, lz = shape
+    n = lx * ly * lz // 2
+    hdr = np.array([lx, ly, lz, CMD[cmd], arg1, arg2, imode, nrep, seed],
+                   dtype=np.int32)
+    par = np.array([beta, beta_a, beta_2, beta_4, lam, alpha, limit_or, delta],
+                   dtype=np.float64)
+    ex1 = np.zeros(n) if ex1 is None else np.asarray(ex1, dtype=np.float64)
+    ex2 = np.zeros(n) if ex2 is None else np.asarray(ex2, dtype=np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(par.tobytes())
+            f.write(np.ascontiguousarray(u, dtype=np.float64).tobytes())
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+            f.write(ex1.tobytes())
+            f.write(ex2.tobytes())
+        env = dict(os.environ)
+        subprocess.run([harness_path(prec), fin, fout], check=True, env=env,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        raw = open(fout, "rb").read()
+    recs = []
+    off = 0
+    while off < len(raw):
+        m = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=off)[0])
+        off += 4
+        recs.append(np.frombuffer(raw, dtype=np.float64, count=m, offset=off).copy())
+        off += 8 * m
+    return recs
