@@ -1,0 +1,158 @@
+"""Throughput of the lattice sweep on one MI355X.
+
+Workload (the metric's 64^3 production configuration): 3D SU(2) + adjoint
+Higgs on a 64^3 lattice in FP64, hot start (Haar links, Higgs components
+N(0, 0.5^2)), couplings beta_G = 9, beta_A = 0.33, beta_2 = -0.9, beta_4 = 0.1
+(trace convention; reference normalisation beta_A/2, beta_2/2, beta_4/4).
+One step = one sweep = 1 Kennedy-Pendleton heatbath link sweep + 4 link
+overrelaxation sweeps (each 6 checkerboard x direction launches, with the
+Higgs-hopping Metropolis) + 1 Higgs Metropolis sweep (2 parities, device-tuned
+step), with plaquette / <rho> / <rho^2> / <a.F> accumulated every sweep.
+
+    python bench.py [--steps K] [--warmup W] [--lattice 64,64,64] [--cpu-seconds S]
+
+Prints one JSON line.  value = site-updates/s (V sites per sweep).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "site-updates/s & HBM GB/s at 64³; indep. samples/s learned vs plain"
+HBM_PEAK_GBPS = 8000.0
+# algorithmic bytes per lattice site for one (parity, mu) link launch:
+# U_mu(eo) read+write 32, U_mu(oe) read 16, U_nu both parities for two nu 64,
+# Higgs both parities 24  ->  136 B per site of the lattice (272 B per link)
+LINK_PASS_BYTES_PER_SITE = 136
+HIGGS_PASS_BYTES_PER_SITE = 132
+SWEEP_BYTES_PER_SITE = 6 * LINK_PASS_BYTES_PER_SITE * 5 + 2 * HIGGS_PASS_BYTES_PER_SITE
+
+
+def couplings(beta_a_trace=0.33):
+    from lceqcd_amd import couplings_from_trace_convention
+    return couplings_from_trace_convention(9.0, beta_a_trace, -0.9, 0.1)
+
+
+def cpu_baseline(shape, seconds, n_or):
+    """The CPU restatement (single thread) on the same workload, bounded."""
+    from oracle.oracle import OracleLattice
+    O = OracleLattice(*shape)
+    O.set_couplings(*couplings())
+    O.hotstart(12345, 0.5)
+    O.philox(12345, 0)
+    d = 0.4
+    sweeps = 0
+    t0 = time.perf_counter()
+    while True:
+        O.agauge(1)
+        for _ in range(n_or):
+            O.link_or(1)
+        d, _ = O.ahiggs1(d, 1)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or sweeps >= 200:
+            break
+    V = shape[0] * shape[1] * shape[2]
+    return {"value": V * sweeps / el, "unit": "site-updates/s", "cores": 1, "kind": "port",
+            "sample": "%d sweeps (1 KP + %d OR + ahiggs1) of the %dx%dx%d workload, %.1f s, "
+                      "oracle/lceq_oracle.c single-threaded" % (sweeps, n_or, *shape, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lattice", default="64,64,64")
+    ap.add_argument("--n-or", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(1)
+
+    import lceqcd_amd
+    shape = tuple(int(x) for x in args.lattice.split(","))
+    V = shape[0] * shape[1] * shape[2]
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(*couplings())
+    L.hotstart(12345, 0.5)
+    L.rng_philox(12345, 0)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.warmup, n_or=args.n_or, delta=0.4, measflag=1)
+    L.history(args.warmup)
+    L.sync()
+
+    # timed region: K sweeps, synchronised on both sides
+    t0 = time.perf_counter()
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.steps, n_or=args.n_or, delta=d, measflag=1)
+    L.sync()
+    t1 = time.perf_counter()
+    hist = L.history(args.steps)
+    ms = (t1 - t0) * 1e3 / args.steps
+
+    # the same K sweeps again with HIP events around every launch of the
+    # dominant kernel (the link-overrelaxation class, 6 * n_or launches per sweep)
+    nlaunch = 6 * args.n_or * args.steps
+    L.profile_begin(lceqcd_amd.KC_LINK_OR, nlaunch)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.steps, n_or=args.n_or, delta=d, measflag=1)
+    or_ms, or_n = L.profile_end()
+    L.history(args.steps)
+    L.profile_begin(lceqcd_amd.KC_LINK_KP, 6 * args.steps)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.steps, n_or=args.n_or, delta=d, measflag=1)
+    kp_ms, kp_n = L.profile_end()
+    L.history(args.steps)
+    avg_s = or_ms * 1e-3 / max(or_n, 1)
+    launch_bytes = LINK_PASS_BYTES_PER_SITE * V
+    achieved = launch_bytes / avg_s / 1e9
+
+    rec = {
+        "metric": METRIC,
+        "value": V / (ms * 1e-3),
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%dx%dx%d SU(2)+adjoint-Higgs sweep: 1 KP heatbath + %d link OR + 1 Higgs "
+                        "Metropolis, hot start, beta_G=9 beta_A=0.33 beta_2=-0.9 beta_4=0.1 "
+                        "(trace convention)" % (shape + (args.n_or,)),
+            "lattice": list(shape),
+            "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
+            "link_updates_per_s": 3 * V * (1 + args.n_or) / (ms * 1e-3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_link<false,false> (link overrelaxation + hopping Metropolis)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": launch_bytes,
+            "avg_launch_us": avg_s * 1e6,
+            "launches": or_n,
+            "kp_avg_launch_us": kp_ms * 1e3 / max(kp_n, 1),
+        },
+        "observables_last_sweep": {"plaq0": hist[-1, 0], "a02": hist[-1, 2],
+                                   "a04": hist[-1, 3], "a0kin": hist[-1, 1],
+                                   "higgs_accept": hist[-1, 8], "delta": d},
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds, args.n_or)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

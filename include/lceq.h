@@ -1,0 +1,177 @@
+/*
+ * lceq.h -- C ABI of the MI355X (gfx950) lattice update and measurement
+ * library liblceq.so.  Every hot-path entry point replaces one Fortran
+ * subroutine of derek-teaney/lceqcd; its comment cites the routine (file:line)
+ * and the calls it replaces.  Fortran callers bind these with iso_c_binding
+ * (lceqcd_amd/fortran/lceq_c.f90); the reference's implicit-interface
+ * arguments map one to one (INTEGER -> int, REAL under -fdefault-real-8 ->
+ * double).
+ *
+ * The lattice is device resident.  The reference keeps the fields in COMMON
+ * /linkvar/ (f/INCLUDE/gaugefield.inc:71-86); here they move between host and
+ * device only through lceq_put_* / lceq_get_*, which take arrays in that COMMON
+ * layout with an explicit leading site dimension ld (the reference's maxarray,
+ * or N = V/2):
+ *   links  u(2 complex, ld, 0:1, 3)  -> double u[((mu*2 + p)*ld + s)*4 + c],
+ *          c = Re alpha, Im alpha, Re beta, Im beta, mu = 0..2, p = 0 "even"
+ *   higgs  a(3, ld, 0:1)             -> double a[(p*ld + s)*3 + i]
+ * The staple v and the force follow the same conventions.
+ *
+ * All functions return LCEQ_OK (0) or a negative status; lceq_last_error()
+ * describes the last failure on the calling thread.  One lattice handle must
+ * not be used from two threads at once.  Work is queued on the handle's own
+ * HIP stream; functions that return host values synchronise that stream.
+ */
+#ifndef LCEQ_H
+#define LCEQ_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LCEQ_OK 0
+#define LCEQ_ERR_ARG (-1)         /* invalid argument or lattice shape */
+#define LCEQ_ERR_HIP (-2)         /* HIP runtime failure */
+#define LCEQ_ERR_RNG (-3)         /* fed uniform stream exhausted */
+#define LCEQ_ERR_NOT_CONVERGED (-4) /* landau_gauge hit max_iter */
+
+/* uniform sources */
+#define LCEQ_RNG_FED 0
+#define LCEQ_RNG_PHILOX 1
+
+/* sweep schedules for lceq_sweep */
+#define LCEQ_SWEEP_KP_OR_METRO 0  /* 1 KP heatbath + n_or link OR + ahiggs1 */
+#define LCEQ_SWEEP_REFERENCE 1    /* ahiggs2, ahiggs3, agauge, renorm (f/main.f:37-41) */
+
+/* kernel classes for lceq_profile_begin */
+#define LCEQ_KC_LINK_KP 1
+#define LCEQ_KC_LINK_OR 2
+#define LCEQ_KC_HIGGS 3
+#define LCEQ_KC_GFIX 4
+
+typedef struct lceq_lattice lceq_lattice;
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* Whole lattice lx*ly*lz on the current HIP device; extents even and >= 2
+ * (f/topology.f:124-142).  Replaces initialize/topology/adresses
+ * (f/initialize.f:35-45, f/adresses.f:18) for one device. */
+int lceq_create(int lx, int ly, int lz, lceq_lattice **out);
+int lceq_destroy(lceq_lattice *L);
+const char *lceq_last_error(void);
+int lceq_sync(lceq_lattice *L);
+
+/* ---- parameters (COMMON /paramet/, f/INCLUDE/input_parameter.inc:66-69) */
+/* beta, beta_a, beta_2, beta_4 in the reference normalisation
+ * (f/paraminit.f:150-153 derives them from beta, x, y). */
+int lceq_set_couplings(lceq_lattice *L, double beta, double beta_a, double beta_2,
+                       double beta_4);
+/* lambda (overrelaxation), alpha (z weight), limit_or; max_iter caps
+ * landau_gauge, which has no cap in the reference (f/landau_gauge.f:54-59). */
+int lceq_set_gauge_fixing(lceq_lattice *L, double lambda, double alpha, double limit_or,
+                          int max_iter);
+/* imode of f/ahiggs2.f:52-77 (0 plain, 1 alpha-corrected width) */
+int lceq_set_imode(lceq_lattice *L, int imode);
+
+/* ---- uniforms --------------------------------------------------------- */
+/* Counter-based Philox4x32-10 streams: draw k of site s in update pass P uses
+ * counter (s, k/2, P_lo, P_hi) and key seed.  Each link (parity, mu) class and
+ * each Higgs parity is one pass; pass_counter is the next pass number. */
+int lceq_rng_philox(lceq_lattice *L, uint64_t seed, uint64_t pass_counter);
+/* Fed mode: the routines consume `uniforms` in exactly the order the
+ * reference's random_number calls do (f/heat_kpi.f:33-93, f/agauge.f:167-169,
+ * f/gauss.f:21-26, f/ahiggs1.f:51-83, f/ahiggs2.f:83-111, f/ahiggs3.f:109-111). */
+int lceq_rng_feed(lceq_lattice *L, const double *uniforms, int64_t n);
+int lceq_rng_state(lceq_lattice *L, int64_t *fed_consumed, uint64_t *pass_counter);
+
+/* ---- fields ----------------------------------------------------------- */
+int lceq_put_links(lceq_lattice *L, const double *u, int64_t ld);
+int lceq_get_links(lceq_lattice *L, double *u, int64_t ld);
+int lceq_put_higgs(lceq_lattice *L, const double *a, int64_t ld);
+int lceq_get_higgs(lceq_lattice *L, double *a, int64_t ld);
+int lceq_put_links_fixed(lceq_lattice *L, const double *u_f, int64_t ld);
+int lceq_get_links_fixed(lceq_lattice *L, double *u_f, int64_t ld);
+int lceq_put_higgs_fixed(lceq_lattice *L, const double *a_f, int64_t ld);
+int lceq_get_higgs_fixed(lceq_lattice *L, double *a_f, int64_t ld);
+/* staple v of the last lceq_staples call (COMMON /helpfields/ v) */
+int lceq_get_staple(lceq_lattice *L, double *v, int64_t ld);
+/* force(3, ld) of the last lceq_a_force call (COMMON /helpfields/ force) */
+int lceq_get_force(lceq_lattice *L, double *force, int64_t ld);
+/* unitconf, f/confget.f:295-307 */
+int lceq_unitconf(lceq_lattice *L);
+/* testconf, f/confget.f:528-575 (one rank) */
+int lceq_testconf(lceq_lattice *L);
+/* hot start: Haar links and N(0, sigma^2) Higgs components (Philox stream) */
+int lceq_hotstart(lceq_lattice *L, uint64_t seed, double sigma);
+
+/* ---- hot path ----------------------------------------------------------- */
+/* staples(eo,oe,mu), f/staples.f:9 (called at f/agauge.f:65); mu = 1..3 */
+int lceq_staples(lceq_lattice *L, int eo, int mu);
+/* heat_kpi(b,v,a0_old,nacch), f/heat_kpi.f:9 (called at f/agauge.f:93) on n
+ * host sites; v gets 4 doubles (Re a, Im a, Re b, Im b) per site. */
+int lceq_heat_kpi(lceq_lattice *L, int n, const double *b, double *v, const double *a0_old,
+                  int *nacch);
+/* agauge(measflag,accept), f/agauge.f:7 (called at f/main.f:23,40) */
+int lceq_agauge(lceq_lattice *L, int measflag, double *accept);
+/* link overrelaxation U -> Vhat^+ U^+ Vhat^+ with the hopping Metropolis of
+ * f/agauge.f:115-174; no counterpart in the reference.  accept = Metropolis
+ * acceptance (0..1). */
+int lceq_link_or(lceq_lattice *L, int measflag, double *accept);
+/* a_force(eo), f/a_force.f:7 (called at f/ahiggs1.f:47, f/ahiggs2.f:48,
+ * f/ahiggs3.f:48) */
+int lceq_a_force(lceq_lattice *L, int eo);
+/* gauss(X,R,NH), f/gauss.f:10 (called at f/ahiggs2.f:83-85) on host arrays */
+int lceq_gauss(lceq_lattice *L, double *x, const double *r, int nh);
+/* ahiggs1(delta,measflag,accept), f/ahiggs1.f:7 */
+int lceq_ahiggs1(lceq_lattice *L, double *delta, int measflag, double *accept);
+/* ahiggs2(measflag,accept), f/ahiggs2.f:7 (called at f/main.f:21,38) */
+int lceq_ahiggs2(lceq_lattice *L, int measflag, double *accept);
+/* ahiggs3(measflag,accept), f/ahiggs3.f:7 (called at f/main.f:22,39) */
+int lceq_ahiggs3(lceq_lattice *L, int measflag, double *accept);
+/* renorm(), f/renorm.f:6 (called at f/main.f:24,41) */
+int lceq_renorm(lceq_lattice *L);
+/* u_f_renorm(), f/u_f_renorm.f:6 (called at f/landau_gauge.f:57) */
+int lceq_u_f_renorm(lceq_lattice *L);
+/* or_gauge_fixing(da1), f/or_gauge_fixing1.f:5 (called at f/landau_gauge.f:56) */
+int lceq_or_gauge_fixing(lceq_lattice *L, double *da1);
+/* landau_gauge(da1,iter_or1), f/landau_gauge.f:6 (called at f/main.f:62,74):
+ * copies u->u_f, a->a_f and iterates or_gauge_fixing + u_f_renorm while
+ * da1 > limit_or.  Returns LCEQ_ERR_NOT_CONVERGED if max_iter is reached. */
+int lceq_landau_gauge(lceq_lattice *L, double *da1, int *iter_or1);
+/* correlator(meas,mstart), f/correlator.f:11 (called at f/meas2a.f:35):
+ * the three z-correlators the reference writes (arrays of length lz); also
+ * sets a0f (f/meas2a.f:39-48) for lceq_observables. */
+int lceq_correlator(lceq_lattice *L, double *corr_a, double *corr_u, double *corr_3);
+/* scalarcorr(meas,mstart), f/scalarcorr.f:10 (called at f/main.f:70) */
+int lceq_scalarcorr(lceq_lattice *L, double *mean, double *corr);
+/* polyakov(meas,mstart), f/polyakov.f:10 (called at f/main.f:69,78) */
+int lceq_polyakov(lceq_lattice *L, double *mean_sq, double *corr);
+/* MULTALL{NN,DN,ND,DD,AD,AN}, f/matmul.f:1-194 (kind 0..5) on host arrays;
+ * s1/s2 are 0-based site maps or NULL (normalsites). */
+int lceq_multall(lceq_lattice *L, int kind, const double *op1, const double *op2, double *out,
+                 int n, const int *s1, const int *s2);
+/* COMMON /aver_loc/ (f/INCLUDE/measurement.inc:30): out[0..6] = plaq0,
+ * a0kin, a02, a04, a0f(1..3) as set by the last routine that writes each. */
+int lceq_observables(lceq_lattice *L, double *out);
+
+/* ---- production sweeps (device resident, no host synchronisation) ------ */
+/* nsweep sweeps of `schedule`; with measflag = 1 each sweep appends one row
+ * of 16 doubles (plaq0, a0kin, a02, a04, a0f[3], link accept, Higgs accept,
+ * da, iter, delta, ...) to the history.  delta (schedule 0) is the ahiggs1
+ * step, tuned on device as f/ahiggs1.f:117-123 does; on return it holds the
+ * current value.  The schedule-0 link writes are renormalised. */
+int lceq_sweep(lceq_lattice *L, int schedule, int nsweep, int n_or, double *delta,
+               int measflag);
+/* copy the last n history rows (n <= rows recorded) and clear the history */
+int lceq_history(lceq_lattice *L, double *out, int n);
+
+/* ---- timing (HIP events on the lattice stream) -------------------------- */
+/* bracket every launch of one kernel class with a pair of events until
+ * lceq_profile_end, which returns their summed duration and count */
+int lceq_profile_begin(lceq_lattice *L, int kernel_class, int max_launches);
+int lceq_profile_end(lceq_lattice *L, double *total_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

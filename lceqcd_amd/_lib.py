@@ -1,0 +1,88 @@
+"""ctypes binding of liblceq.so (include/lceq.h)."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblceq.so")
+
+D, I, I64, U64, VP = C.c_double, C.c_int, C.c_int64, C.c_uint64, C.c_void_p
+DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "lceq_create": [I, I, I, C.POINTER(VP)],
+    "lceq_destroy": [VP],
+    "lceq_sync": [VP],
+    "lceq_set_couplings": [VP, D, D, D, D],
+    "lceq_set_gauge_fixing": [VP, D, D, D, I],
+    "lceq_set_imode": [VP, I],
+    "lceq_rng_philox": [VP, U64, U64],
+    "lceq_rng_feed": [VP, DP, I64],
+    "lceq_rng_state": [VP, C.POINTER(I64), C.POINTER(U64)],
+    "lceq_put_links": [VP, DP, I64],
+    "lceq_get_links": [VP, DP, I64],
+    "lceq_put_higgs": [VP, DP, I64],
+    "lceq_get_higgs": [VP, DP, I64],
+    "lceq_put_links_fixed": [VP, DP, I64],
+    "lceq_get_links_fixed": [VP, DP, I64],
+    "lceq_put_higgs_fixed": [VP, DP, I64],
+    "lceq_get_higgs_fixed": [VP, DP, I64],
+    "lceq_get_staple": [VP, DP, I64],
+    "lceq_get_force": [VP, DP, I64],
+    "lceq_unitconf": [VP],
+    "lceq_testconf": [VP],
+    "lceq_hotstart": [VP, U64, D],
+    "lceq_staples": [VP, I, I],
+    "lceq_heat_kpi": [VP, I, DP, DP, DP, IP],
+    "lceq_agauge": [VP, I, DP],
+    "lceq_link_or": [VP, I, DP],
+    "lceq_a_force": [VP, I],
+    "lceq_gauss": [VP, DP, DP, I],
+    "lceq_ahiggs1": [VP, DP, I, DP],
+    "lceq_ahiggs2": [VP, I, DP],
+    "lceq_ahiggs3": [VP, I, DP],
+    "lceq_renorm": [VP],
+    "lceq_u_f_renorm": [VP],
+    "lceq_or_gauge_fixing": [VP, DP],
+    "lceq_landau_gauge": [VP, DP, IP],
+    "lceq_correlator": [VP, DP, DP, DP],
+    "lceq_scalarcorr": [VP, DP, DP],
+    "lceq_polyakov": [VP, DP, DP],
+    "lceq_multall": [VP, I, DP, DP, DP, I, IP, IP],
+    "lceq_observables": [VP, DP],
+    "lceq_sweep": [VP, I, I, I, DP, I],
+    "lceq_history": [VP, DP, I],
+    "lceq_profile_begin": [VP, I, I],
+    "lceq_profile_end": [VP, DP, IP],
+}
+
+_lib = None
+
+
+class LceqError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree liblceq.so; fails loudly when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LceqError(
+                "liblceq.so is not built; run `python -m lceqcd_amd.build` "
+                "(there is no CPU fallback)")
+        lib = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = I
+        lib.lceq_last_error.argtypes = []
+        lib.lceq_last_error.restype = C.c_char_p
+        _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().lceq_last_error().decode(errors="replace")
+        raise LceqError("liblceq status %d: %s" % (rc, msg))

@@ -1,0 +1,38 @@
+"""Build liblceq.so in-tree with hipcc for gfx950.
+
+    python -m lceqcd_amd.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = [os.path.join(HERE, "csrc", f) for f in ("lceq_kernels.hip", "lceq_api.hip")]
+OUT = os.path.join(HERE, "liblceq.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
+    # keep the compiler away from packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unknown-warning-option",
+]
+
+
+def build(verbose=False):
+    """Compile every HIP source into lceqcd_amd/liblceq.so (always rebuilds)."""
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC] + FLAGS + SRC + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building liblceq.so")
+    if verbose:
+        sys.stderr.write(r.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,210 @@
+// lceq_device.hpp -- device building blocks for the gfx950 lattice kernels:
+// SU(2) algebra in (alpha, beta) form, the checkerboard geometry, the
+// Philox4x32-10 uniform streams, and deterministic wavefront/LDS reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lceq {
+
+// ---------------------------------------------------------------- complex
+
+struct cpx {
+  double re, im;
+};
+__device__ __forceinline__ cpx c_mk(double r, double i) { return cpx{r, i}; }
+__device__ __forceinline__ cpx c_add(cpx a, cpx b) { return cpx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cpx c_sub(cpx a, cpx b) { return cpx{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cpx c_neg(cpx a) { return cpx{-a.re, -a.im}; }
+__device__ __forceinline__ cpx c_conj(cpx a) { return cpx{a.re, -a.im}; }
+__device__ __forceinline__ cpx c_mul(cpx a, cpx b) {
+  return cpx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cpx c_scale(double t, cpx a) { return cpx{t * a.re, t * a.im}; }
+
+// U = [[alpha, beta], [-beta*, alpha*]]; the same form holds for sums of
+// SU(2) matrices (staples) and for the algebra element X(a) = i a.sigma,
+// (alpha, beta) = (i a3, a2 + i a1)  (f/INCLUDE/gaugefield.inc:89-101).
+struct su2 {
+  cpx a, b;
+};
+
+// products in the operand conventions of f/matmul.f
+__device__ __forceinline__ su2 m_nn(su2 p, su2 q) {  // U1 U2   (matmul.f:22-23)
+  return su2{c_sub(c_mul(p.a, q.a), c_mul(p.b, c_conj(q.b))),
+             c_add(c_mul(p.a, q.b), c_mul(p.b, c_conj(q.a)))};
+}
+__device__ __forceinline__ su2 m_dn(su2 p, su2 q) {  // U1^+ U2 (matmul.f:52-56)
+  return su2{c_add(c_mul(c_conj(p.a), q.a), c_mul(p.b, c_conj(q.b))),
+             c_sub(c_mul(c_conj(p.a), q.b), c_mul(p.b, c_conj(q.a)))};
+}
+__device__ __forceinline__ su2 m_nd(su2 p, su2 q) {  // U1 U2^+ (matmul.f:85-89)
+  return su2{c_add(c_mul(p.a, c_conj(q.a)), c_mul(p.b, c_conj(q.b))),
+             c_add(c_neg(c_mul(p.a, q.b)), c_mul(p.b, q.a))};
+}
+__device__ __forceinline__ su2 m_dd(su2 p, su2 q) {  // U1^+ U2^+ (matmul.f:118-122)
+  return su2{c_sub(c_mul(c_conj(p.a), c_conj(q.a)), c_mul(p.b, c_conj(q.b))),
+             c_sub(c_neg(c_mul(c_conj(p.a), q.b)), c_mul(p.b, q.a))};
+}
+struct vec3 {
+  double x[3];
+};
+__device__ __forceinline__ su2 m_ad(const vec3& a, su2 u) {  // X(a) U^+ (matmul.f:150-154)
+  cpx x1 = c_mk(0.0, a.x[2]), x2 = c_mk(a.x[1], a.x[0]);
+  return su2{c_add(c_mul(x1, c_conj(u.a)), c_mul(x2, c_conj(u.b))),
+             c_add(c_neg(c_mul(x1, u.b)), c_mul(x2, u.a))};
+}
+__device__ __forceinline__ su2 m_an(const vec3& a, su2 u) {  // X(a) U (matmul.f:183-187)
+  cpx x1 = c_mk(0.0, a.x[2]), x2 = c_mk(a.x[1], a.x[0]);
+  return su2{c_sub(c_mul(x1, u.a), c_mul(x2, c_conj(u.b))),
+             c_add(c_mul(x1, u.b), c_mul(x2, c_conj(u.a)))};
+}
+
+// ------------------------------------------------------------ geometry
+
+// Checkerboard numbering of f/adresses.f on one rank: a site (x,y,z), 0-based,
+// has parity p = (x+y+z)&1 (p = 0 is the reference's "even") and index
+// s = x/2 + h1*y + h2*z within its parity, h1 = lx/2, h2 = lx*ly/2.
+struct Geom {
+  int lx, ly, lz;
+  int h1, h2, n;  // n = sites per parity
+  int npad;       // leading dimension of every SoA component array
+};
+
+struct Coord {
+  int x, y, z;
+};
+
+__device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
+  int z = s / g.h2;
+  int r = s - z * g.h2;
+  int y = r / g.h1;
+  int xh = r - y * g.h1;
+  return Coord{2 * xh + ((p + y + z) & 1), y, z};
+}
+__device__ __forceinline__ int site_index(const Geom& g, int x, int y, int z) {
+  return (x >> 1) + g.h1 * y + g.h2 * z;
+}
+__device__ __forceinline__ int wrap(int c, int l) { return c < 0 ? c + l : (c >= l ? c - l : c); }
+// index of (c + d), d a displacement with components in {-1,0,1}
+__device__ __forceinline__ int shifted(const Geom& g, Coord c, int dx, int dy, int dz) {
+  return site_index(g, wrap(c.x + dx, g.lx), wrap(c.y + dy, g.ly), wrap(c.z + dz, g.lz));
+}
+__device__ __forceinline__ void unit(int mu, int& dx, int& dy, int& dz) {
+  dx = mu == 0;
+  dy = mu == 1;
+  dz = mu == 2;
+}
+
+// ------------------------------------------------------------- storage
+
+// Links: component c of U_mu at parity p, site s at
+//   U[((mu*2 + p)*4 + c)*npad + s],   c = Re alpha, Im alpha, Re beta, Im beta.
+// Higgs: A[(p*3 + i)*npad + s].
+__device__ __forceinline__ su2 ld_link(const double* __restrict__ U, const Geom& g, int mu,
+                                       int p, int s) {
+  const double* q = U + (size_t)((mu * 2 + p) * 4) * g.npad + s;
+  return su2{c_mk(q[0], q[g.npad]), c_mk(q[2 * (size_t)g.npad], q[3 * (size_t)g.npad])};
+}
+__device__ __forceinline__ void st_link(double* __restrict__ U, const Geom& g, int mu, int p,
+                                        int s, su2 u) {
+  double* q = U + (size_t)((mu * 2 + p) * 4) * g.npad + s;
+  q[0] = u.a.re;
+  q[g.npad] = u.a.im;
+  q[2 * (size_t)g.npad] = u.b.re;
+  q[3 * (size_t)g.npad] = u.b.im;
+}
+__device__ __forceinline__ vec3 ld_higgs(const double* __restrict__ A, const Geom& g, int p,
+                                         int s) {
+  const double* q = A + (size_t)(p * 3) * g.npad + s;
+  return vec3{{q[0], q[g.npad], q[2 * (size_t)g.npad]}};
+}
+__device__ __forceinline__ void st_higgs(double* __restrict__ A, const Geom& g, int p, int s,
+                                         const vec3& a) {
+  double* q = A + (size_t)(p * 3) * g.npad + s;
+  q[0] = a.x[0];
+  q[g.npad] = a.x[1];
+  q[2 * (size_t)g.npad] = a.x[2];
+}
+
+// ----------------------------------------------------------------- RNG
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Draw k of lattice
+// index idx in pass `pass` uses counter (idx, (stream<<8) | k/2, pass_lo,
+// pass_hi) and key = seed; words 2(k%2), 2(k%2)+1 give a 52-bit x and the
+// uniform (2x+1) 2^-53 in (0,1).
+struct Philox {
+  uint32_t k0, k1;
+};
+
+__device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
+  uint64_t x = ((uint64_t)hi << 20) | (uint64_t)(lo >> 12);
+  return (double)(2 * x + 1) * 0x1p-53;
+}
+// the two uniforms 2j, 2j+1 of (idx, stream, pass)
+__device__ __forceinline__ void philox_pair(const Philox& key, uint32_t idx, uint32_t stream,
+                                            uint64_t pass, int j, double& ua, double& ub) {
+  uint32_t w[4];
+  philox_block(idx, (stream << 8) | (uint32_t)j, (uint32_t)pass, (uint32_t)(pass >> 32), key.k0,
+               key.k1, w);
+  ua = u01(w[0], w[1]);
+  ub = u01(w[2], w[3]);
+}
+
+// ------------------------------------------------------------ reductions
+
+// Deterministic block sum of NQ doubles (blockDim = 256 = 4 waves of 64):
+// xor-butterfly inside each wave, then the four wave totals in fixed order.
+template <int NQ>
+__device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restrict__ out,
+                                                int stride, int blk) {
+  __shared__ double red[4][NQ > 0 ? NQ : 1];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double t = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    v[q] = t;
+  }
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) red[wv][q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    int q = threadIdx.x;
+    out[(size_t)q * stride + blk] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  }
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; hand each XCD a contiguous
+// range of logical blocks so that neighbour gathers (z-neighbours are
+// lx*ly/2 sites away) stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int nb) {
+  int i = blockIdx.x;
+  int k = i & 7, j = i >> 3;
+  int q = nb >> 3, r = nb & 7;
+  return k * q + (k < r ? k : r) + j;
+}
+
+}  // namespace lceq

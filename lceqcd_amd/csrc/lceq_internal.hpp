@@ -1,0 +1,95 @@
+// lceq_internal.hpp -- the lattice context shared by the C-ABI layer
+// (lceq_api.hip) and the kernel launchers (lceq_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lceq_device.hpp"
+
+// number of per-block partial quantities a reduction slot holds
+#define LCEQ_NQ 4
+// reduction slots per routine call (6 link sub-passes, 2 Higgs parities, ...)
+#define LCEQ_NSLOT 8
+
+// device observables (double obs[LCEQ_OBS_N])
+enum {
+  OBS_PLAQ0 = 0,   // plaq0 of the last link update (f/agauge.f:179-200)
+  OBS_A0KIN = 1,   // a0kin of the last Higgs update (f/ahiggs2.f:126-147)
+  OBS_A02 = 2,
+  OBS_A04 = 3,
+  OBS_A0F = 4,     // 4..6: a0f(1..3) (f/meas2a.f:39-48)
+  OBS_ACC_LINK = 7,
+  OBS_ACC_HIGGS = 8,
+  OBS_DA = 9,
+  OBS_ITER = 10,
+  OBS_DELTA = 11,  // ahiggs1 Metropolis step (device-resident in sweeps)
+  OBS_NACCH = 12,  // raw counts of the last link call
+  OBS_NACC1 = 13,
+  OBS_GF_DONE = 14,  // landau_gauge convergence flag (1.0 = converged)
+  OBS_GF_CAPPED = 15,
+  LCEQ_OBS_N = 16
+};
+
+struct lceq_lattice {
+  lceq::Geom g;
+  int vol;
+  int device;
+  hipStream_t stream;
+  // device fields
+  double *U, *A, *Uf, *Af, *Vst, *F;
+  double *partials;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
+  int pstride;       // >= number of 256-site blocks
+  double *obs;       // [LCEQ_OBS_N]
+  double *hist;      // [hist_cap][LCEQ_OBS_N] per-sweep records
+  int hist_cap;
+  int hist_rows;     // rows recorded since the last lceq_history
+  double *slices;    // correlator slice sums [14][lz]
+  double *corr;      // correlator outputs [5][lz] + 4 scalars
+  double *scratch;   // generic device scratch for host-array entry points
+  size_t scratch_bytes;
+  double *fed;       // fed uniforms
+  int64_t fed_cap, fed_n, fed_pos;
+  // parameters (reference COMMON /paramet/, f/INCLUDE/input_parameter.inc:66-69)
+  double beta, beta_a, beta_2, beta_4, lambda, alpha, limit_or;
+  int max_iter, imode;
+  // RNG
+  int rng_mode;  // 0 fed, 1 philox
+  uint64_t seed, pass;
+  // per-launch timing of one kernel class (bench instrumentation)
+  int prof_class;
+  int prof_n, prof_cap;
+  hipEvent_t *prof_ev;
+};
+
+namespace lceq {
+
+enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
+
+int blocks_for(int n);
+
+// launchers (lceq_kernels.hip); all enqueue on L->stream, no host sync
+void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
+                 const double* fed, uint64_t pass, int slot);
+void launch_link_finalize(lceq_lattice* L, int kp, int measflag);
+void launch_staples(lceq_lattice* L, int eo, int mu);
+void launch_force(lceq_lattice* L, int eo);
+void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
+                  uint64_t pass, int slot);
+void launch_higgs_finalize(lceq_lattice* L, int kind, int measflag);
+void launch_renorm(lceq_lattice* L, double* U);
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int slot);
+void launch_gfix_finalize(lceq_lattice* L, int guarded);
+void launch_slices(lceq_lattice* L);
+void launch_corr(lceq_lattice* L);
+void launch_heat_kpi(lceq_lattice* L, int n, const double* b, const double* a0_old, double* v,
+                     const double* fed, uint64_t pass);
+void launch_gauss(lceq_lattice* L, int n, const double* r, double* x, const double* fed,
+                  uint64_t pass);
+void launch_multall(lceq_lattice* L, int kind, const double* op1, const double* op2,
+                    double* out, int n, const int* s1, const int* s2);
+void launch_hotstart(lceq_lattice* L, uint64_t seed, double sigma);
+void launch_unitconf(lceq_lattice* L);
+void launch_testconf(lceq_lattice* L);
+void launch_record(lceq_lattice* L, int index);
+
+}  // namespace lceq

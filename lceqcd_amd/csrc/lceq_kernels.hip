@@ -1,0 +1,941 @@
+// lceq_kernels.hip -- gfx950 kernels of the lattice update and measurement
+// path.  One thread owns one site of the parity being updated; a (parity, mu)
+// link class or a Higgs parity is one launch, so the checkerboard replaces the
+// reference's MPI halo exchanges and barriers.  All arithmetic is FP64.
+#include <math.h>
+
+#include "lceq_internal.hpp"
+
+namespace lceq {
+
+static constexpr int TB = 256;
+static constexpr double TPI = 2.0 * M_PI;  // 2*acos(-1.0), f/heat_kpi.f:29
+
+int blocks_for(int n) { return (n + TB - 1) / TB; }
+
+static inline void prof_begin(lceq_lattice* L, int kc) {
+  if (L->prof_class == kc && L->prof_n + 2 <= L->prof_cap)
+    (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
+}
+static inline void prof_end(lceq_lattice* L, int kc) {
+  if (L->prof_class == kc && (L->prof_n & 1)) (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
+}
+
+// deterministic sum of x[0..n) by one 256-thread block; result in all threads
+__device__ double block_sum_range(const double* __restrict__ x, int n) {
+  __shared__ double red[4];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += TB) t += x[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  double r = ((red[0] + red[1]) + red[2]) + red[3];
+  __syncthreads();
+  return r;
+}
+
+// ================================================================ links
+
+struct LinkArgs {
+  Geom g;
+  double* U;
+  const double* A;
+  int eo, mu;
+  double beta, beta_a;
+  const double* fed;  // fed uniforms of this (eo, mu) block, reference order
+  Philox key;
+  uint64_t pass;
+  int measflag, renorm;
+  double* part;  // partials of this slot: [LCEQ_NQ][pstride]
+  int pstride, nb;
+};
+
+// staple V(x) of f/staples.f:65-216 for x in parity eo, direction mu
+__device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& g, int eo, int mu,
+                                      int s, Coord c) {
+  int oe = eo ^ 1;
+  int mx, my, mz;
+  unit(mu, mx, my, mz);
+  int sxm = shifted(g, c, mx, my, mz);  // x+mu, parity oe
+  su2 V{c_mk(0.0, 0.0), c_mk(0.0, 0.0)};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    int nu = (t == 0) ? (mu == 0 ? 1 : 0) : (mu == 2 ? 1 : 2);
+    int nx, ny, nz;
+    unit(nu, nx, ny, nz);
+    int sxn = shifted(g, c, nx, ny, nz);                      // x+nu   (oe)
+    int sxmn = shifted(g, c, -nx, -ny, -nz);                  // x-nu   (oe)
+    int sxmmn = shifted(g, c, mx - nx, my - ny, mz - nz);     // x+mu-nu (eo)
+    su2 up1 = m_dd(ld_link(U, g, mu, oe, sxn), ld_link(U, g, nu, eo, s));     // 3^+ 2^+
+    su2 up2 = m_nn(ld_link(U, g, nu, oe, sxm), up1);                          // 4 3^+ 2^+
+    su2 bt1 = m_dn(ld_link(U, g, mu, oe, sxmn), ld_link(U, g, nu, oe, sxmn)); // 6^+ 5
+    su2 bt2 = m_dn(ld_link(U, g, nu, eo, sxmmn), bt1);                        // 7^+ 6^+ 5
+    V.a = c_add(c_add(V.a, up2.a), bt2.a);
+    V.b = c_add(c_add(V.b, up2.b), bt2.b);
+  }
+  return V;
+}
+
+__device__ __forceinline__ double hop_action(double beta_a, const vec3& ax, const vec3& axp,
+                                             su2 u) {
+  return beta_a * m_nn(m_an(ax, u), m_ad(axp, u)).a.re;  // f/agauge.f:126-159
+}
+
+__device__ __forceinline__ su2 normalized(su2 u) {
+  double t = (u.a.re * u.a.re + u.a.im * u.a.im) + (u.b.re * u.b.re + u.b.im * u.b.im);
+  t = 1.0 / sqrt(t);
+  return su2{c_scale(t, u.a), c_scale(t, u.b)};
+}
+
+// One (eo, mu) class of f/agauge.f:53-193 (KP=true), or the link
+// overrelaxation U -> Vhat^+ U^+ Vhat^+ with the same hopping Metropolis
+// (KP=false).  Partials: 0 plaquette, 1 KP accepts, 2 Metropolis accepts.
+template <bool KP, bool FED>
+__global__ __launch_bounds__(256) void k_link(LinkArgs p) {
+  const Geom& g = p.g;
+  int blk = xcd_block(p.nb);
+  int s = blk * TB + threadIdx.x;
+  double red[3] = {0.0, 0.0, 0.0};
+  if (s < g.n) {
+    int eo = p.eo, oe = eo ^ 1, mu = p.mu;
+    Coord c = site_coord(g, eo, s);
+    int mx, my, mz;
+    unit(mu, mx, my, mz);
+    int sxm = shifted(g, c, mx, my, mz);
+    su2 V = staple(p.U, g, eo, mu, s, c);
+    su2 u = ld_link(p.U, g, mu, eo, s);
+    vec3 ax = ld_higgs(p.A, g, eo, s), axp = ld_higgs(p.A, g, oe, sxm);
+    // f/agauge.f:71-84
+    double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
+    t = 1.0 / sqrt(t);
+    su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
+    su2 h;
+    double racc;
+    if (KP) {
+      double b = t / p.beta;
+      double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
+      double r1, r2, r3, r4, r5, r6;
+      if (FED) {
+        const double* f = p.fed;
+        int n = g.n;
+        r1 = f[3 * s];
+        r2 = f[3 * s + 1];
+        r3 = f[3 * s + 2];
+        r4 = f[3 * n + s];
+        r5 = f[4 * n + s];
+        r6 = f[5 * n + s];
+        racc = f[6 * n + s];
+      } else {
+        double dummy;
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, r1, r2);
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, r3, r4);
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 2, r5, r6);
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 3, racc, dummy);
+      }
+      // Kennedy-Pendleton, f/heat_kpi.f:42-98
+      if (r1 == 0.0) r1 = 0.0000033;
+      if (r2 == 0.0) r2 = 0.0000066;
+      double cc = cos(TPI * r3);
+      double a0 = -(log(r1) + log(r2) * (cc * cc)) * b;
+      bool lacc = r4 * r4 <= (1.0 - 0.5 * a0);
+      red[1] = lacc ? 1.0 : 0.0;
+      a0 = lacc ? 1.0 - a0 : a0old;
+      double a1 = 1.0 - a0 * a0;
+      a1 = a1 >= 0.0 ? sqrt(a1) : 0.0;
+      double a2 = 2.0 * (r5 - 0.5);
+      su2 w;
+      w.a = c_mk(a0, a1 * a2);
+      double tt = 1.0 - a2 * a2;
+      double a0b = tt >= 0.0 ? a1 * sqrt(tt) : 0.0;
+      double ang = TPI * r6;
+      w.b = c_mk(a0b * sin(ang), a0b * cos(ang));
+      h = m_nd(w, vh);  // f/agauge.f:98
+    } else {
+      if (FED) {
+        racc = p.fed[s];
+      } else {
+        double dummy;
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, racc, dummy);
+      }
+      su2 r = m_nn(m_nn(vh, u), vh);
+      h = su2{c_conj(r.a), c_neg(r.b)};
+    }
+    double sa_o = hop_action(p.beta_a, ax, axp, u);
+    double sa_n = hop_action(p.beta_a, ax, axp, h);
+    if (exp(sa_o - sa_n) >= racc) {  // f/agauge.f:167-174
+      u = p.renorm ? normalized(h) : h;
+      st_link(p.U, g, mu, eo, s, u);
+      red[2] = 1.0;
+    }
+    if (p.measflag == 1) red[0] = m_nn(u, V).a.re;  // f/agauge.f:179-185
+  }
+  block_sum_store<3>(red, p.part, p.pstride, blk);
+}
+
+// Combine the six (eo, mu) slots in reference order (f/agauge.f:195-200).
+__global__ __launch_bounds__(256) void k_link_finalize(const double* __restrict__ part, int pstride,
+                                                       int nb, double vol, int kp, int measflag,
+                                                       double* __restrict__ obs) {
+  double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
+  for (int slot = 0; slot < 6; ++slot) {
+    const double* ps = part + (size_t)slot * LCEQ_NQ * pstride;
+    double e = block_sum_range(ps, nb);
+    double k = block_sum_range(ps + pstride, nb);
+    double m = block_sum_range(ps + 2 * (size_t)pstride, nb);
+    plaq = plaq + e;
+    nacch += k;
+    acc1 += m;
+  }
+  if (threadIdx.x == 0) {
+    obs[OBS_NACCH] = nacch;
+    obs[OBS_NACC1] = acc1;
+    if (kp) {
+      double accept = nacch;
+      if (measflag == 1) {
+        obs[OBS_PLAQ0] = plaq / (12.0 * vol);
+        accept = accept / (3.0 * vol);
+        accept = accept * (acc1 / (12.0 * vol));
+      }
+      obs[OBS_ACC_LINK] = accept;
+    } else {
+      if (measflag == 1) obs[OBS_PLAQ0] = plaq / (12.0 * vol);
+      obs[OBS_ACC_LINK] = acc1 / (3.0 * vol);
+    }
+  }
+}
+
+void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
+                 const double* fed, uint64_t pass, int slot) {
+  LinkArgs a;
+  a.g = L->g;
+  a.U = L->U;
+  a.A = L->A;
+  a.eo = eo;
+  a.mu = mu;
+  a.beta = L->beta;
+  a.beta_a = L->beta_a;
+  a.fed = fed;
+  a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
+  a.pass = pass;
+  a.measflag = measflag;
+  a.renorm = renorm_on_write;
+  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
+  a.pstride = L->pstride;
+  a.nb = blocks_for(L->g.n);
+  int kc = kp ? KC_LINK_KP : KC_LINK_OR;
+  prof_begin(L, kc);
+  if (kp) {
+    if (fed)
+      hipLaunchKernelGGL((k_link<true, true>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+    else
+      hipLaunchKernelGGL((k_link<true, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+  } else {
+    if (fed)
+      hipLaunchKernelGGL((k_link<false, true>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+    else
+      hipLaunchKernelGGL((k_link<false, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+  }
+  prof_end(L, kc);
+}
+
+void launch_link_finalize(lceq_lattice* L, int kp, int measflag) {
+  hipLaunchKernelGGL(k_link_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials, L->pstride,
+                     blocks_for(L->g.n), (double)L->vol, kp, measflag, L->obs);
+}
+
+// staple only, for the staples() entry point (f/staples.f:9)
+__global__ __launch_bounds__(256) void k_staples(Geom g, const double* __restrict__ U,
+                                                 double* __restrict__ Vst, int eo, int mu) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  if (s >= g.n) return;
+  Coord c = site_coord(g, eo, s);
+  st_link(Vst, g, mu, eo, s, staple(U, g, eo, mu, s, c));
+}
+
+void launch_staples(lceq_lattice* L, int eo, int mu) {
+  hipLaunchKernelGGL(k_staples, dim3(blocks_for(L->g.n)), dim3(TB), 0, L->stream, L->g, L->U,
+                     L->Vst, eo, mu);
+}
+
+// ================================================================ Higgs
+
+// F(x) of f/a_force.f:54-110 for x in parity eo
+__device__ __forceinline__ vec3 hop_force(const double* __restrict__ U,
+                                          const double* __restrict__ A, const Geom& g, int eo,
+                                          int s, Coord c) {
+  int oe = eo ^ 1;
+  vec3 F{{0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    int mx, my, mz;
+    unit(mu, mx, my, mz);
+    int sxp = shifted(g, c, mx, my, mz), sxd = shifted(g, c, -mx, -my, -mz);
+    su2 ux = ld_link(U, g, mu, eo, s);
+    su2 t1 = m_nn(ux, m_ad(ld_higgs(A, g, oe, sxp), ux));
+    su2 uy = ld_link(U, g, mu, oe, sxd);
+    su2 t2 = m_dn(uy, m_an(ld_higgs(A, g, oe, sxd), uy));
+    F.x[2] = F.x[2] + c_add(t1.a, t2.a).im;
+    F.x[1] = F.x[1] + c_add(t1.b, t2.b).re;
+    F.x[0] = F.x[0] + c_add(t1.b, t2.b).im;
+  }
+  return F;
+}
+
+__global__ __launch_bounds__(256) void k_force(Geom g, const double* __restrict__ U,
+                                               const double* __restrict__ A,
+                                               double* __restrict__ F, int eo) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  if (s >= g.n) return;
+  vec3 f = hop_force(U, A, g, eo, s, site_coord(g, eo, s));
+  F[s] = f.x[0];
+  F[g.npad + s] = f.x[1];
+  F[2 * (size_t)g.npad + s] = f.x[2];
+}
+
+void launch_force(lceq_lattice* L, int eo) {
+  hipLaunchKernelGGL(k_force, dim3(blocks_for(L->g.n)), dim3(TB), 0, L->stream, L->g, L->U, L->A,
+                     L->F, eo);
+}
+
+struct HiggsArgs {
+  Geom g;
+  const double* U;
+  double* A;
+  int eo, imode;
+  double beta_a, beta_2, beta_4;
+  const double* delta;  // device scalar (ahiggs1)
+  const double* fed;
+  Philox key;
+  uint64_t pass;
+  int measflag;
+  double* part;
+  int pstride, nb;
+};
+
+// KIND 1: f/ahiggs1.f Metropolis; 2: f/ahiggs2.f heatbath; 3: f/ahiggs3.f
+// overrelaxation.  Partials: 0 accepts, 1 rho, 2 rho^2, 3 a.F (post update).
+template <int KIND, bool FED>
+__global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
+  const Geom& g = p.g;
+  int blk = xcd_block(p.nb);
+  int s = blk * TB + threadIdx.x;
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  if (s < g.n) {
+    int eo = p.eo, n = g.n;
+    Coord c = site_coord(g, eo, s);
+    vec3 F = hop_force(p.U, p.A, g, eo, s, c);
+    vec3 a = ld_higgs(p.A, g, eo, s), an;
+    double ro = 0.0, rn = 0.0, boltz, racc;
+    if (KIND == 1) {  // f/ahiggs1.f:51-92
+      double delta = *p.delta, r[3];
+      if (FED) {
+        r[0] = p.fed[3 * s];
+        r[1] = p.fed[3 * s + 1];
+        r[2] = p.fed[3 * s + 2];
+        racc = p.fed[3 * n + s];
+      } else {
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, r[0], r[1]);
+        double dummy;
+        philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, r[2], racc);
+        (void)dummy;
+      }
+      double sfo = 0.0, sfn = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        an.x[i] = a.x[i] + (r[i] - 0.5) * delta;
+        sfo = sfo + a.x[i] * F.x[i];
+        sfn = sfn + an.x[i] * F.x[i];
+        ro = ro + a.x[i] * a.x[i];
+        rn = rn + an.x[i] * an.x[i];
+      }
+      boltz = exp(-(p.beta_a * (sfo - sfn)) + p.beta_2 * (ro - rn) +
+                  p.beta_4 * (ro * ro - rn * rn));
+    } else {
+      double alph, width, cx;
+      if (p.imode == 1) {  // f/ahiggs2.f:61-77
+        double conf2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) conf2 = conf2 + p.beta_a * (F.x[i] * F.x[i]);
+        double d = p.beta_2 + p.beta_4 * conf2;
+        alph = (p.beta_4 * conf2) / (2.0 * (d * d));
+        width = p.beta_2 + alph;
+        cx = 1.0 / 2.0 / width;
+      } else {  // f/ahiggs2.f:52-56
+        alph = 0.0;
+        width = p.beta_2;
+        cx = 1.0 / p.beta_2 / 2.0;
+      }
+      if (KIND == 2) {  // gauss x3 (f/gauss.f:18-29), f/ahiggs2.f:83-91
+        double w[6];
+        if (FED) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) w[k] = p.fed[(size_t)k * n + s];
+          racc = p.fed[6 * (size_t)n + s];
+        } else {
+          double dummy;
+          philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, w[0], w[1]);
+          philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, w[2], w[3]);
+          philox_pair(p.key, (uint32_t)s, 0, p.pass, 2, w[4], w[5]);
+          philox_pair(p.key, (uint32_t)s, 0, p.pass, 3, racc, dummy);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double w1 = -1.0 * (1.0 / width) * log(w[2 * i]);
+          double gi = sqrt(w1) * cos(TPI * w[2 * i + 1]);
+          an.x[i] = gi + p.beta_a * F.x[i] * cx;
+        }
+      } else {  // f/ahiggs3.f:83-87
+        if (FED) {
+          racc = p.fed[s];
+        } else {
+          double dummy;
+          philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, racc, dummy);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) an.x[i] = -a.x[i] + 2.0 * p.beta_a * F.x[i] * cx;
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        ro = ro + a.x[i] * a.x[i];
+        rn = rn + an.x[i] * an.x[i];
+      }
+      boltz = exp(-(alph * (ro - rn)) + p.beta_4 * (ro * ro - rn * rn));
+    }
+    if (boltz >= racc) {
+      a = an;
+      ro = rn;
+      st_higgs(p.A, g, eo, s, a);
+      red[0] = 1.0;
+    }
+    if (p.measflag == 1) {
+      red[1] = ro;
+      red[2] = ro * ro;
+      red[3] = (a.x[0] * F.x[0] + a.x[1] * F.x[1]) + a.x[2] * F.x[2];
+    }
+  }
+  block_sum_store<4>(red, p.part, p.pstride, blk);
+}
+
+// f/ahiggs1.f:107-123, f/ahiggs2.f:310-315
+__global__ __launch_bounds__(256) void k_higgs_finalize(const double* __restrict__ part, int pstride,
+                                                        int nb, double vol, int kind, int measflag,
+                                                        double* __restrict__ obs) {
+  double acc = 0.0, r2 = 0.0, r4 = 0.0, kin = 0.0;
+  for (int slot = 0; slot < 2; ++slot) {
+    const double* ps = part + (size_t)slot * LCEQ_NQ * pstride;
+    acc += block_sum_range(ps, nb);
+    r2 = r2 + block_sum_range(ps + pstride, nb);
+    r4 = r4 + block_sum_range(ps + 2 * (size_t)pstride, nb);
+    kin = kin + block_sum_range(ps + 3 * (size_t)pstride, nb);
+  }
+  if (threadIdx.x == 0) {
+    if (measflag == 1) {
+      obs[OBS_A02] = r2 / vol;
+      obs[OBS_A04] = r4 / vol;
+      obs[OBS_A0KIN] = kin / (6.0 * vol);
+    }
+    double accept = acc / vol;
+    if (kind == 1) {
+      obs[OBS_ACC_HIGGS] = accept;
+      double d = obs[OBS_DELTA];
+      if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+      obs[OBS_DELTA] = d;
+    } else if (measflag == 1) {
+      obs[OBS_ACC_HIGGS] = accept;
+    }
+  }
+}
+
+void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
+                  uint64_t pass, int slot) {
+  HiggsArgs a;
+  a.g = L->g;
+  a.U = L->U;
+  a.A = L->A;
+  a.eo = eo;
+  a.imode = L->imode;
+  a.beta_a = L->beta_a;
+  a.beta_2 = L->beta_2;
+  a.beta_4 = L->beta_4;
+  a.delta = L->obs + OBS_DELTA;
+  a.fed = fed;
+  a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
+  a.pass = pass;
+  a.measflag = measflag;
+  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
+  a.pstride = L->pstride;
+  a.nb = blocks_for(L->g.n);
+  dim3 grid(a.nb), blk(TB);
+  prof_begin(L, KC_HIGGS);
+#define LCEQ_HIGGS_LAUNCH(K)                                                         \
+  if (fed)                                                                           \
+    hipLaunchKernelGGL((k_higgs<K, true>), grid, blk, 0, L->stream, a);              \
+  else                                                                               \
+    hipLaunchKernelGGL((k_higgs<K, false>), grid, blk, 0, L->stream, a);
+  if (kind == 1) {
+    LCEQ_HIGGS_LAUNCH(1)
+  } else if (kind == 2) {
+    LCEQ_HIGGS_LAUNCH(2)
+  } else {
+    LCEQ_HIGGS_LAUNCH(3)
+  }
+#undef LCEQ_HIGGS_LAUNCH
+  prof_end(L, KC_HIGGS);
+}
+
+void launch_higgs_finalize(lceq_lattice* L, int kind, int measflag) {
+  hipLaunchKernelGGL(k_higgs_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials, L->pstride,
+                     blocks_for(L->g.n), (double)L->vol, kind, measflag, L->obs);
+}
+
+// =============================================================== renorm
+
+// f/renorm.f:24-34 / f/u_f_renorm.f:93-103 over all 3V links
+__global__ __launch_bounds__(256) void k_renorm(Geom g, double* __restrict__ U) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  int cls = blockIdx.y;  // mu*2 + p
+  if (s >= g.n) return;
+  int mu = cls >> 1, p = cls & 1;
+  st_link(U, g, mu, p, s, normalized(ld_link(U, g, mu, p, s)));
+}
+
+void launch_renorm(lceq_lattice* L, double* U) {
+  hipLaunchKernelGGL(k_renorm, dim3(blocks_for(L->g.n), 6), dim3(TB), 0, L->stream, L->g, U);
+}
+
+// ========================================================= gauge fixing
+
+struct GfArgs {
+  Geom g;
+  double* Uf;
+  double* Af;
+  int eo, compute_da, renorm, guarded;
+  double lambda, alpha;
+  const double* obs;  // OBS_GF_DONE read when guarded
+  double* part;
+  int pstride, nb;
+};
+
+// One parity half of f/or_gauge_fixing1.f:54-328.  Thread x reads and
+// writes only U_mu(x) and U_mu(x-mu) for mu = 1..3 and a_f(x), so the half
+// step has no cross-thread hazards.  With renorm=1 (eo = 1 of a
+// landau_gauge iteration) every link of the lattice is written exactly once
+// and f/u_f_renorm.f is applied on that write.
+__global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
+  if (p.guarded && p.obs[OBS_GF_DONE] != 0.0) return;  // uniform over the grid
+  const Geom& g = p.g;
+  int blk = xcd_block(p.nb);
+  int s = blk * TB + threadIdx.x;
+  double red[1] = {0.0};
+  if (s < g.n) {
+    int eo = p.eo, oe = eo ^ 1;
+    Coord c = site_coord(g, eo, s);
+    su2 ux[3], uy[3];
+    int sy[3];
+#pragma unroll
+    for (int mu = 0; mu < 3; ++mu) {
+      int mx, my, mz;
+      unit(mu, mx, my, mz);
+      sy[mu] = shifted(g, c, -mx, -my, -mz);
+      ux[mu] = ld_link(p.Uf, g, mu, eo, s);
+      uy[mu] = ld_link(p.Uf, g, mu, oe, sy[mu]);
+    }
+    // :97-107, :116-142
+    su2 s1{c_add(ux[0].a, ux[1].a), c_add(ux[0].b, ux[1].b)};
+    s1.a = c_add(s1.a, c_scale(p.alpha, ux[2].a));
+    s1.b = c_add(s1.b, c_scale(p.alpha, ux[2].b));
+    su2 s2{c_add(uy[0].a, uy[1].a), c_add(uy[0].b, uy[1].b)};
+    s2.a = c_add(s2.a, c_scale(p.alpha, uy[2].a));
+    s2.b = c_add(s2.b, c_scale(p.alpha, uy[2].b));
+    su2 w{c_add(s1.a, c_conj(s2.a)), c_sub(s1.b, s2.b)};  // :145-146
+    if (p.compute_da) {                                    // :153-158
+      double d = c_sub(s1.a, s2.a).im;
+      red[0] = d * d + (w.b.re * w.b.re + w.b.im * w.b.im);
+    }
+    double k = (w.a.re * w.a.re + w.a.im * w.a.im) + (w.b.re * w.b.re + w.b.im * w.b.im);
+    double sk = sqrt(k);  // :169-183
+    su2 o{c_mk(w.a.re / sk, w.a.im / sk), c_mk(w.b.re / sk, w.b.im / sk)};
+    double ra = fmin(1.0, fmax(-1.0, o.a.re));
+    double th = acos(ra);
+    if (th != 0.0) {
+      double tho = p.lambda * th;
+      double f = sin(tho) / sin(th);
+      double t2 = f * o.a.im;
+      o.a = c_mk(cos(tho), -t2);
+      o.b = c_mk(-(f * o.b.re), -(f * o.b.im));
+    }
+    vec3 af = ld_higgs(p.Af, g, eo, s);  // :243-252
+    su2 t = m_nn(o, m_ad(af, o));
+    st_higgs(p.Af, g, eo, s, vec3{{t.b.im, t.b.re, t.a.im}});
+#pragma unroll
+    for (int mu = 0; mu < 3; ++mu) {  // :256-324
+      su2 nx = m_nn(o, ux[mu]);
+      su2 ny = m_nd(uy[mu], o);
+      if (p.renorm) {
+        nx = normalized(nx);
+        ny = normalized(ny);
+      }
+      st_link(p.Uf, g, mu, eo, s, nx);
+      st_link(p.Uf, g, mu, oe, sy[mu], ny);
+    }
+  }
+  if (p.compute_da) block_sum_store<1>(red, p.part, p.pstride, blk);
+}
+
+// da1 = sum/N (f/or_gauge_fixing1.f:156-157); for guarded (landau_gauge)
+// iterations also count the iteration and set the convergence flag
+// (f/landau_gauge.f:54-59) so later launches of the batch exit at once.
+__global__ __launch_bounds__(256) void k_gfix_finalize(const double* __restrict__ part, int nb,
+                                                       double n, int guarded, double limit_or,
+                                                       double max_iter, double* __restrict__ obs) {
+  if (guarded && obs[OBS_GF_DONE] != 0.0) return;
+  double da = block_sum_range(part, nb) / n;
+  if (threadIdx.x == 0) {
+    obs[OBS_DA] = da;
+    if (guarded) {
+      double it = obs[OBS_ITER] + 1.0;
+      obs[OBS_ITER] = it;
+      if (!(da > limit_or)) obs[OBS_GF_DONE] = 1.0;
+      else if (it >= max_iter) {
+        obs[OBS_GF_DONE] = 1.0;
+        obs[OBS_GF_CAPPED] = 1.0;
+      }
+    }
+  }
+}
+
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int slot) {
+  GfArgs a;
+  a.g = L->g;
+  a.Uf = L->Uf;
+  a.Af = L->Af;
+  a.eo = eo;
+  a.compute_da = compute_da;
+  a.renorm = renorm;
+  a.guarded = guarded;
+  a.lambda = L->lambda;
+  a.alpha = L->alpha;
+  a.obs = L->obs;
+  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
+  a.pstride = L->pstride;
+  a.nb = blocks_for(L->g.n);
+  prof_begin(L, KC_GFIX);
+  hipLaunchKernelGGL(k_gfix, dim3(a.nb), dim3(TB), 0, L->stream, a);
+  prof_end(L, KC_GFIX);
+}
+
+void launch_gfix_finalize(lceq_lattice* L, int guarded) {
+  hipLaunchKernelGGL(k_gfix_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials,
+                     blocks_for(L->g.n), (double)L->g.n, guarded, L->limit_or,
+                     (double)L->max_iter, L->obs);
+}
+
+// ========================================================== correlators
+
+// z-slice sums over both parities (f/correlator.f:53-79, f/scalarcorr.f:43-56,
+// f/polyakov.f:172-185).  Quantities q (slices[q*lz + z]):
+//   0-2 a_f(1..3); 3+3*mu: Im alpha_mu, 4+3*mu: Im beta_mu, 5+3*mu: -Re beta_mu
+//   (the (Im a, -i b) projection of u_f); 12 rho(a); 13 cos(4|a|).
+__global__ __launch_bounds__(256) void k_slices(Geom g, const double* __restrict__ Uf,
+                                                const double* __restrict__ Af,
+                                                const double* __restrict__ A,
+                                                double* __restrict__ slices) {
+  __shared__ double red[4][14];
+  int z = blockIdx.x;
+  double acc[14];
+#pragma unroll
+  for (int q = 0; q < 14; ++q) acc[q] = 0.0;
+  for (int i = threadIdx.x; i < 2 * g.h2; i += TB) {
+    int p = i >= g.h2, s = z * g.h2 + (i - p * g.h2);
+    vec3 af = ld_higgs(Af, g, p, s), a = ld_higgs(A, g, p, s);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] += af.x[k];
+#pragma unroll
+    for (int mu = 0; mu < 3; ++mu) {
+      su2 u = ld_link(Uf, g, mu, p, s);
+      acc[3 + 3 * mu] += u.a.im;
+      acc[4 + 3 * mu] += u.b.im;
+      acc[5 + 3 * mu] -= u.b.re;
+    }
+    double rho = a.x[0] * a.x[0] + a.x[1] * a.x[1] + a.x[2] * a.x[2];
+    acc[12] += rho;
+    acc[13] += cos(4.0 * sqrt(rho));
+  }
+#pragma unroll
+  for (int q = 0; q < 14; ++q) {
+    double t = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 14) {
+    int q = threadIdx.x;
+    slices[q * g.lz + z] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  }
+}
+
+// Periodic z-correlators on the slice means (f/correlator.f:139-164,
+// f/scalarcorr.f:96-104, f/polyakov.f:225-233).  corr layout:
+// [0,lz) corr_a, [lz,2lz) corr_u, [2lz,3lz) corr_3, [3lz,4lz) rho corr,
+// [4lz,5lz) polyakov corr, 5lz: mean rho, 5lz+1: mean(P)^2, 5lz+2..4: a0f.
+__global__ __launch_bounds__(256) void k_corr(Geom g, double* __restrict__ slices,
+                                              double* __restrict__ corr) {
+  int lz = g.lz;
+  double area = (double)(g.lx * g.ly);
+  for (int i = threadIdx.x; i < 14 * lz; i += TB) slices[i] = slices[i] / area;
+  __syncthreads();
+  const double* as = slices;
+  for (int z = threadIdx.x; z < lz; z += TB) {
+    double ca = 0.0, cu = 0.0, c3 = 0.0, cr = 0.0, cp = 0.0;
+    for (int x0 = 0; x0 < lz; ++x0) {
+      int x1 = x0 + z;
+      if (x1 >= lz) x1 -= lz;
+      double d = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d = d + as[k * lz + x0] * as[k * lz + x1];
+      ca = ca + d;
+      // Re[A1 A1' + A2 conj(A2')] for mu = 1,2 and mu = 3, A1 real
+      double t = 0.0;
+#pragma unroll
+      for (int mu = 0; mu < 2; ++mu) {
+        double a1 = as[(3 + 3 * mu) * lz + x0], a1p = as[(3 + 3 * mu) * lz + x1];
+        double br = as[(4 + 3 * mu) * lz + x0], bi = as[(5 + 3 * mu) * lz + x0];
+        double brp = as[(4 + 3 * mu) * lz + x1], bip = as[(5 + 3 * mu) * lz + x1];
+        t = t + a1 * a1p;
+        t = t + (br * brp + bi * bip);
+      }
+      cu = cu + t;
+      {
+        double a1 = as[9 * lz + x0], a1p = as[9 * lz + x1];
+        double br = as[10 * lz + x0], bi = as[11 * lz + x0];
+        double brp = as[10 * lz + x1], bip = as[11 * lz + x1];
+        c3 = c3 + (a1 * a1p + (br * brp + bi * bip));
+      }
+      cr = cr + as[12 * lz + x0] * as[12 * lz + x1];
+      cp = cp + as[13 * lz + x0] * as[13 * lz + x1];
+    }
+    corr[z] = ca / lz;
+    corr[lz + z] = cu / (lz * 2);
+    corr[2 * lz + z] = c3 / lz;
+    corr[3 * lz + z] = cr / lz;
+    corr[4 * lz + z] = cp / lz;
+  }
+  if (threadIdx.x == 0) {
+    double mr = 0.0, mp = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int z = 0; z < lz; ++z) {
+      mr += as[12 * lz + z];
+      mp += as[13 * lz + z];
+      f0 += as[z];
+      f1 += as[lz + z];
+      f2 += as[2 * lz + z];
+    }
+    mp = mp / lz;
+    corr[5 * lz] = mr / lz;
+    corr[5 * lz + 1] = mp * mp;
+    corr[5 * lz + 2] = f0 / lz;  // a0f(i) = mean over sites = mean of slice means
+    corr[5 * lz + 3] = f1 / lz;
+    corr[5 * lz + 4] = f2 / lz;
+  }
+}
+
+void launch_slices(lceq_lattice* L) {
+  hipLaunchKernelGGL(k_slices, dim3(L->g.lz), dim3(TB), 0, L->stream, L->g, L->Uf, L->Af, L->A,
+                     L->slices);
+}
+void launch_corr(lceq_lattice* L) {
+  hipLaunchKernelGGL(k_corr, dim3(1), dim3(TB), 0, L->stream, L->g, L->slices, L->corr);
+}
+
+// ============================================ array-level entry points
+
+// f/heat_kpi.f:9-101 on n independent sites (b, a0_old in; W out as 4
+// doubles per site in (alpha, beta) form); partial 0 of slot 0 counts accepts.
+__global__ __launch_bounds__(256) void k_heat_kpi(int n, const double* __restrict__ b,
+                                                  const double* __restrict__ a0_old,
+                                                  double* __restrict__ v,
+                                                  const double* __restrict__ fed, Philox key,
+                                                  uint64_t pass, double* __restrict__ part,
+                                                  int nb) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  double red[1] = {0.0};
+  if (s < n) {
+    double r1, r2, r3, r4, r5, r6;
+    if (fed) {
+      r1 = fed[3 * s];
+      r2 = fed[3 * s + 1];
+      r3 = fed[3 * s + 2];
+      r4 = fed[3 * n + s];
+      r5 = fed[4 * n + s];
+      r6 = fed[5 * n + s];
+    } else {
+      philox_pair(key, (uint32_t)s, 0, pass, 0, r1, r2);
+      philox_pair(key, (uint32_t)s, 0, pass, 1, r3, r4);
+      philox_pair(key, (uint32_t)s, 0, pass, 2, r5, r6);
+    }
+    if (r1 == 0.0) r1 = 0.0000033;
+    if (r2 == 0.0) r2 = 0.0000066;
+    double cc = cos(TPI * r3);
+    double a0 = -(log(r1) + log(r2) * (cc * cc)) * b[s];
+    bool lacc = r4 * r4 <= (1.0 - 0.5 * a0);
+    red[0] = lacc ? 1.0 : 0.0;
+    a0 = lacc ? 1.0 - a0 : a0_old[s];
+    double a1 = 1.0 - a0 * a0;
+    a1 = a1 >= 0.0 ? sqrt(a1) : 0.0;
+    double a2 = 2.0 * (r5 - 0.5);
+    double tt = 1.0 - a2 * a2;
+    double a0b = tt >= 0.0 ? a1 * sqrt(tt) : 0.0;
+    double ang = TPI * r6;
+    v[4 * (size_t)s] = a0;
+    v[4 * (size_t)s + 1] = a1 * a2;
+    v[4 * (size_t)s + 2] = a0b * sin(ang);
+    v[4 * (size_t)s + 3] = a0b * cos(ang);
+  }
+  block_sum_store<1>(red, part, nb, blockIdx.x);
+}
+
+void launch_heat_kpi(lceq_lattice* L, int n, const double* b, const double* a0_old, double* v,
+                     const double* fed, uint64_t pass) {
+  int nb = blocks_for(n);
+  hipLaunchKernelGGL(k_heat_kpi, dim3(nb), dim3(TB), 0, L->stream, n, b, a0_old, v, fed,
+                     Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)}, pass, L->partials, nb);
+}
+
+// f/gauss.f:10-31
+__global__ __launch_bounds__(256) void k_gauss(int n, const double* __restrict__ r,
+                                               double* __restrict__ x,
+                                               const double* __restrict__ fed, Philox key,
+                                               uint64_t pass) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  if (s >= n) return;
+  double w1, w2;
+  if (fed) {
+    w1 = fed[s];
+    w2 = fed[n + s];
+  } else {
+    philox_pair(key, (uint32_t)s, 0, pass, 0, w1, w2);
+  }
+  double t = -1.0 * (1.0 / r[s]) * log(w1);
+  x[s] = sqrt(t) * cos(TPI * w2);
+}
+
+void launch_gauss(lceq_lattice* L, int n, const double* r, double* x, const double* fed,
+                  uint64_t pass) {
+  hipLaunchKernelGGL(k_gauss, dim3(blocks_for(n)), dim3(TB), 0, L->stream, n, r, x, fed,
+                     Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)}, pass);
+}
+
+// f/matmul.f MULTALL{NN,DN,ND,DD,AD,AN} on site vectors (kind 0..5); operands
+// in (alpha, beta) quadruples, Higgs triples for AD/AN; s1/s2 may be null.
+__global__ __launch_bounds__(256) void k_multall(int kind, const double* __restrict__ op1,
+                                                 const double* __restrict__ op2,
+                                                 double* __restrict__ out, int n,
+                                                 const int* __restrict__ s1,
+                                                 const int* __restrict__ s2) {
+  int i = blockIdx.x * TB + threadIdx.x;
+  if (i >= n) return;
+  int i1 = s1 ? s1[i] : i, i2 = s2 ? s2[i] : i;
+  const double* q2 = op2 + 4 * (size_t)i2;
+  su2 q{c_mk(q2[0], q2[1]), c_mk(q2[2], q2[3])}, h;
+  if (kind >= 4) {
+    const double* a1 = op1 + 3 * (size_t)i1;
+    vec3 a{{a1[0], a1[1], a1[2]}};
+    h = kind == 4 ? m_ad(a, q) : m_an(a, q);
+  } else {
+    const double* q1 = op1 + 4 * (size_t)i1;
+    su2 pp{c_mk(q1[0], q1[1]), c_mk(q1[2], q1[3])};
+    h = kind == 0 ? m_nn(pp, q) : kind == 1 ? m_dn(pp, q) : kind == 2 ? m_nd(pp, q) : m_dd(pp, q);
+  }
+  double* o = out + 4 * (size_t)i;
+  o[0] = h.a.re;
+  o[1] = h.a.im;
+  o[2] = h.b.re;
+  o[3] = h.b.im;
+}
+
+void launch_multall(lceq_lattice* L, int kind, const double* op1, const double* op2, double* out,
+                    int n, const int* s1, const int* s2) {
+  hipLaunchKernelGGL(k_multall, dim3(blocks_for(n)), dim3(TB), 0, L->stream, kind, op1, op2, out,
+                     n, s1, s2);
+}
+
+// ======================================================= start configs
+
+// Haar links (four Box-Muller normals, stream 1) and N(0, sigma^2) Higgs
+// components (stream 2); counter index = (mu*2+p)*n + s resp. p*n + s.
+__global__ __launch_bounds__(256) void k_hotstart(Geom g, double* __restrict__ U,
+                                                  double* __restrict__ A, Philox key,
+                                                  double sigma) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  int cls = blockIdx.y;  // 0..5 links, 6..7 Higgs
+  if (s >= g.n) return;
+  double u0, u1, u2, u3;
+  if (cls < 6) {
+    uint32_t idx = (uint32_t)(cls * g.n + s);
+    philox_pair(key, idx, 1, 0, 0, u0, u1);
+    philox_pair(key, idx, 1, 0, 1, u2, u3);
+    double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+    double q0 = r0 * cos(TPI * u1), q1 = r0 * sin(TPI * u1);
+    double q2 = r1 * cos(TPI * u3), q3 = r1 * sin(TPI * u3);
+    double nrm = 1.0 / sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+    st_link(U, g, cls >> 1, cls & 1, s, su2{c_mk(q0 * nrm, q1 * nrm), c_mk(q2 * nrm, q3 * nrm)});
+  } else {
+    int p = cls - 6;
+    uint32_t idx = (uint32_t)(p * g.n + s);
+    philox_pair(key, idx, 2, 0, 0, u0, u1);
+    philox_pair(key, idx, 2, 0, 1, u2, u3);
+    double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+    st_higgs(A, g, p, s,
+             vec3{{sigma * r0 * cos(TPI * u1), sigma * r0 * sin(TPI * u1),
+                   sigma * r1 * cos(TPI * u3)}});
+  }
+}
+
+void launch_hotstart(lceq_lattice* L, uint64_t seed, double sigma) {
+  hipLaunchKernelGGL(k_hotstart, dim3(blocks_for(L->g.n), 8), dim3(TB), 0, L->stream, L->g, L->U,
+                     L->A, Philox{(uint32_t)seed, (uint32_t)(seed >> 32)}, sigma);
+}
+
+// f/confget.f:295-307 (mode 0) and f/confget.f:528-575 on one rank (mode 1)
+__global__ __launch_bounds__(256) void k_startconf(Geom g, double* __restrict__ U,
+                                                   double* __restrict__ A, int mode) {
+  int s = blockIdx.x * TB + threadIdx.x;
+  int cls = blockIdx.y;
+  if (s >= g.n) return;
+  if (cls < 6) {
+    int mu = cls >> 1;
+    su2 u;
+    if (mode == 0) {
+      u = su2{c_mk(1.0, 0.0), c_mk(0.0, 0.0)};
+    } else {
+      double z = (double)(s / g.h2) * (mu + 1);
+      u = su2{c_mk(0.0, z), c_mk(0.0, z)};
+    }
+    st_link(U, g, mu, cls & 1, s, u);
+  } else {
+    double c = 1.0 / sqrt(3.0);
+    st_higgs(A, g, cls - 6, s, mode == 0 ? vec3{{0.0, 0.0, 1.0}} : vec3{{c, c, c}});
+  }
+}
+
+void launch_unitconf(lceq_lattice* L) {
+  hipLaunchKernelGGL(k_startconf, dim3(blocks_for(L->g.n), 8), dim3(TB), 0, L->stream, L->g, L->U,
+                     L->A, 0);
+}
+void launch_testconf(lceq_lattice* L) {
+  hipLaunchKernelGGL(k_startconf, dim3(blocks_for(L->g.n), 8), dim3(TB), 0, L->stream, L->g, L->U,
+                     L->A, 1);
+}
+
+// copy the observables into the per-sweep history row `index`
+__global__ void k_record(const double* __restrict__ obs, double* __restrict__ hist, int index) {
+  int i = threadIdx.x;
+  if (i < LCEQ_OBS_N) hist[(size_t)index * LCEQ_OBS_N + i] = obs[i];
+}
+void launch_record(lceq_lattice* L, int index) {
+  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, L->stream, L->obs, L->hist, index);
+}
+
+}  // namespace lceq

@@ -295,3 +295,44 @@ class OracleLattice:
         self._L.oc_observables(self.h, _dp(out))
         return dict(plaq0=out[0], a0kin=out[1], a02=out[2], a04=out[3],
                     a0f=out[4:7].copy())
+
+    # ---- the product's method names (lceqcd_amd.Lattice), so that one parity
+    # test body drives either implementation
+    def rng_feed(self, uniforms):
+        self.feed(uniforms)
+
+    def rng_philox(self, seed, pass_counter=0):
+        self.philox(seed, pass_counter)
+
+    def rng_state(self):
+        return self.consumed(), self.pass_counter()
+
+    def put_links(self, u):
+        self.u[:] = u
+
+    def get_links(self):
+        return self.u.copy()
+
+    def put_higgs(self, a):
+        self.a[:] = a
+
+    def get_higgs(self):
+        return self.a.copy()
+
+    def put_links_fixed(self, u):
+        self.u_f[:] = u
+
+    def get_links_fixed(self):
+        return self.u_f.copy()
+
+    def put_higgs_fixed(self, a):
+        self.a_f[:] = a
+
+    def get_higgs_fixed(self):
+        return self.a_f.copy()
+
+    def get_staple(self):
+        return self.v.copy()
+
+    def get_force(self):
+        return self.force.copy()

@@ -1,0 +1,243 @@
+"""HIP kernels (liblceq.so through the C ABI) against the reference's golden
+vectors and against the CPU restatement, on the same inputs.
+
+Bar: 1e-12 relative in FP64 for fields and observables (tolerances are stated
+per case in tests/parity_cases.py), exact for counts, zero accept/reject flips
+(a flip would show up as an O(1) field difference and fail the bar)."""
+import numpy as np
+import pytest
+
+from helpers import assert_close, rand_conf, relerr
+from parity_cases import all_cases, check_all
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def make():
+    import lceqcd_amd
+    return lceqcd_amd.Lattice
+
+
+@pytest.fixture(scope="module")
+def omake(oracle_lib):
+    return oracle_lib.OracleLattice
+
+
+@pytest.mark.parametrize("cid,case", all_cases(), ids=[c[0] for c in all_cases()])
+def test_golden(make, cid, case):
+    print("golden case", cid, flush=True)
+    check_all(case(make))
+
+
+COUP = (9.0, 0.175, 0.45, 0.025)
+PROD = (9.0, 0.175, -0.45, 0.025)
+
+
+def _pair(make, omake, shape, seed, coup=COUP, asig=0.5):
+    u, a = rand_conf(shape, seed, asig)
+    G, O = make(*shape), omake(*shape)
+    for L in (G, O):
+        L.set_couplings(*coup)
+        L.put_links(u)
+        L.put_higgs(a)
+        L.rng_philox(1000 + seed, 7)
+    return G, O
+
+
+SHAPES = [(2, 2, 2), (2, 4, 6), (4, 4, 4), (6, 4, 8), (8, 8, 8), (16, 16, 16), (8, 8, 24)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
+def test_philox_link_updates(make, omake, shape):
+    """agauge and link overrelaxation in production RNG mode, GPU == oracle."""
+    print("philox links", shape, flush=True)
+    G, O = _pair(make, omake, shape, 1)
+    for k in range(2):
+        for L in (G, O):
+            L._acc = L.agauge(1)
+            L._pl = L.observables()["plaq0"]
+        assert_close(G._acc, O._acc, 1e-12, "agauge accept")
+        assert_close(G._pl, O._pl, 1e-12, "plaq0")
+        for L in (G, O):
+            L._acc = L.link_or(1)
+            L._pl = L.observables()["plaq0"]
+        assert_close(G._acc, O._acc, 1e-12, "link_or accept")
+        assert_close(G._pl, O._pl, 1e-12, "plaq0 (OR)")
+    assert_close(G.get_links(), O.get_links(), 1e-12, "links")
+    assert G.rng_state()[1] == O.rng_state()[1]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
+@pytest.mark.parametrize("imode", [0, 1])
+def test_philox_higgs_updates(make, omake, shape, imode):
+    print("philox higgs", shape, imode, flush=True)
+    G, O = _pair(make, omake, shape, 2)
+    for L in (G, O):
+        L.set_imode(imode)
+    dG = dO = 0.4
+    for k in range(2):
+        dG, aG = G.ahiggs1(dG, 1)
+        dO, aO = O.ahiggs1(dO, 1)
+        assert_close(aG, aO, 1e-12, "ahiggs1 accept")
+        assert dG == dO
+        for fn in ("ahiggs2", "ahiggs3"):
+            aG, aO = getattr(G, fn)(1), getattr(O, fn)(1)
+            assert_close(aG, aO, 1e-12, fn + " accept")
+            oG, oO = G.observables(), O.observables()
+            for key in ("a02", "a04", "a0kin"):
+                assert_close(oG[key], oO[key], 1e-11, "%s %s" % (fn, key))
+    assert_close(G.get_higgs(), O.get_higgs(), 1e-12, "higgs")
+
+
+def test_philox_production_sweeps_16(make, omake):
+    """Five sweeps of 1 KP + 4 OR + ahiggs1 at the production couplings."""
+    shape = (16, 16, 16)
+    G, O = _pair(make, omake, shape, 3, coup=PROD, asig=2.0)
+    d = 0.4
+    for it in range(5):
+        O.agauge(0)
+        for k in range(4):
+            O.link_or(0)
+        d, _ = O.ahiggs1(d, 1)
+    dG = G.sweep(0, 5, n_or=4, delta=0.4, measflag=1)
+    assert dG == d
+    # the device sweep renormalises on write; compare after renormalising the oracle
+    O.renorm()
+    assert_close(G.get_links(), O.get_links(), 1e-11, "links after 5 sweeps")
+    assert_close(G.get_higgs(), O.get_higgs(), 1e-11, "higgs after 5 sweeps")
+    h = G.history(5)
+    assert_close(h[-1, 2], O.observables()["a02"], 1e-11, "a02")
+
+
+def test_reference_schedule_sweeps(make, omake):
+    shape = (8, 8, 8)
+    G, O = _pair(make, omake, shape, 4)
+    for it in range(3):
+        O.ahiggs2(1)
+        O.ahiggs3(1)
+        O.agauge(1)
+        O.renorm()
+    G.sweep(1, 3, measflag=1)
+    h = G.history(3)
+    assert_close(G.get_links(), O.get_links(), 1e-11, "links")
+    assert_close(G.get_higgs(), O.get_higgs(), 1e-11, "higgs")
+    assert_close(h[-1, 0], O.observables()["plaq0"], 1e-11, "plaq0")
+
+
+@pytest.mark.parametrize("shape,alpha", [((8, 8, 8), 1.0), ((6, 4, 8), 0.7), ((16, 16, 16), 1.0)])
+def test_gauge_fixing(make, omake, shape, alpha):
+    print("gauge fixing", shape, alpha, flush=True)
+    G, O = _pair(make, omake, shape, 5)
+    u, a = O.get_links(), O.get_higgs()
+    for L in (G, O):
+        L.set_gauge_fixing(1.7, alpha, 1e-10, 100000)
+        L.put_links_fixed(u)
+        L.put_higgs_fixed(a)
+    for k in range(4):
+        dG, dO = G.or_gauge_fixing(), O.or_gauge_fixing()
+        G.u_f_renorm()
+        O.u_f_renorm()
+        assert_close(dG, dO, 1e-11, "da")
+    assert_close(G.get_links_fixed(), O.get_links_fixed(), 1e-12, "u_f")
+    assert_close(G.get_higgs_fixed(), O.get_higgs_fixed(), 1e-12, "a_f")
+    daG, itG = G.landau_gauge()
+    daO, itO = O.landau_gauge()
+    assert abs(itG - itO) <= 1 and daG <= 1e-10
+    if itG == itO:
+        assert_close(G.get_links_fixed(), O.get_links_fixed(), 1e-9, "u_f landau")
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 4, 8), (16, 16, 32)])
+def test_correlators(make, omake, shape):
+    G, O = _pair(make, omake, shape, 6)
+    uf, af = rand_conf(shape, 60)
+    for L in (G, O):
+        L.put_links_fixed(uf)
+        L.put_higgs_fixed(af)
+    for got, exp in zip(G.correlator(), O.correlator()):
+        assert_close(got, exp, 1e-12, "correlator")
+    assert_close(G.observables()["a0f"], O.observables()["a0f"], 1e-11, "a0f")
+    mG, cG = G.scalarcorr()
+    mO, cO = O.scalarcorr()
+    assert_close(mG, mO, 1e-12)
+    assert_close(cG, cO, 1e-12)
+    mG, cG = G.polyakov()
+    mO, cO = O.polyakov()
+    assert_close(mG, mO, 1e-12)
+    assert_close(cG, cO, 1e-12)
+
+
+def test_start_configs(make, omake):
+    shape = (6, 4, 8)
+    G, O = make(*shape), omake(*shape)
+    for fn in ("unitconf", "testconf"):
+        getattr(G, fn)()
+        getattr(O, fn)()
+        assert np.array_equal(G.get_links(), O.get_links()), fn
+        assert np.array_equal(G.get_higgs(), O.get_higgs()), fn
+    G.hotstart(99, 0.5)
+    O.hotstart(99, 0.5)
+    assert_close(G.get_links(), O.get_links(), 1e-14, "hot links")
+    assert_close(G.get_higgs(), O.get_higgs(), 1e-14, "hot higgs")
+
+
+def test_multall(make, oracle_lib):
+    rng = np.random.default_rng(7)
+    n = 300
+    op1 = rng.normal(size=(n, 4))
+    op2 = rng.normal(size=(n, 4))
+    ah = rng.normal(size=(n, 3))
+    s1 = rng.permutation(n).astype(np.int32)
+    s2 = rng.permutation(n).astype(np.int32)
+    G = make(4, 4, 4)
+    for kind in ("nn", "dn", "nd", "dd"):
+        assert_close(G.multall(kind, op1, op2, s1, s2), oracle_lib.multall(kind, op1, op2, s1, s2),
+                     1e-14, kind)
+    for kind in ("ad", "an"):
+        assert_close(G.multall(kind, ah, op2, s1, None), oracle_lib.multall(kind, ah, op2, s1, None),
+                     1e-14, kind)
+
+
+def test_fed_stream_exhaustion_is_an_error(make):
+    import lceqcd_amd
+    G = make(4, 4, 4)
+    G.unitconf()
+    G.set_couplings(*COUP)
+    G.rng_feed(np.full(10, 0.5))
+    with pytest.raises(lceqcd_amd.LceqError):
+        G.agauge(1)
+
+
+def test_determinism_64(make):
+    """Same seed, same result, bit for bit, on the full 64^3 lattice."""
+    out = []
+    for rep in range(2):
+        G = make(64, 64, 64)
+        G.set_couplings(*PROD)
+        G.hotstart(12345, 0.5)
+        G.rng_philox(12345)
+        G.sweep(0, 2, n_or=4, delta=0.4, measflag=1)
+        out.append((G.get_links(), G.get_higgs(), G.history(2)))
+        G.close()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
+def test_unitarity_and_or_invariance_64(make):
+    """Size-independent properties at 64^3: unit links after production
+    sweeps; with beta_A = 0 a link-overrelaxation sweep is always accepted
+    and leaves the Wilson action (brute-force plaquette sum) unchanged."""
+    from helpers import plaquette_sum
+    G = make(64, 64, 64)
+    G.set_couplings(9.0, 0.0, 0.45, 0.025)
+    G.hotstart(7, 0.5)
+    G.rng_philox(7)
+    G.sweep(0, 2, n_or=4, delta=0.4, measflag=1)
+    u = G.get_links()
+    assert np.max(np.abs(np.sum(u * u, axis=-1) - 1.0)) < 1e-14
+    s0 = plaquette_sum((64, 64, 64), u)
+    acc = G.link_or(1)
+    s1 = plaquette_sum((64, 64, 64), G.get_links())
+    assert acc == 1.0
+    assert abs(s1 - s0) <= 1e-12 * abs(s0)
