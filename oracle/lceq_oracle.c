@@ -111,6 +111,7 @@ struct oc_lattice {
   double beta, beta_a, beta_2, beta_4;
   double lambda, alpha, limit_or;
   int max_iter, imode;
+  int renorm_on_write; /* production sweeps: normalise accepted links */
   int rng_mode;
   const double *fed;
   int64_t fed_n, fed_pos;
@@ -302,6 +303,7 @@ void oc_set_gauge_fixing(oc_lattice *L, double lambda, double alpha,
   L->max_iter = max_iter;
 }
 void oc_set_imode(oc_lattice *L, int imode) { L->imode = imode; }
+void oc_set_renorm_on_write(oc_lattice *L, int flag) { L->renorm_on_write = flag; }
 void oc_rng_feed(oc_lattice *L, const double *buf, int64_t n) {
   L->rng_mode = OC_RNG_FED;
   L->fed = buf;
@@ -523,6 +525,8 @@ void oc_heat_kpi(oc_lattice *L, int n, const double *b, double *v,
   free(w);
 }
 
+static void renorm_field(double *u, size_t nlinks);
+
 /* ----------------------------------------------------- hop term helpers */
 
 /* beta_A Re[(X(a_x) U)(X(a_{x+mu}) U^+)]_11, f/agauge.f:126-159 */
@@ -585,6 +589,7 @@ static void link_update(oc_lattice *L, int kp, int measflag, double *accept) {
       for (int s = 0; s < n; ++s) { /* :167-174 */
         double rx = fed ? oc_draw(L) : oc_pu(L, pass, s, kp ? 6 : 0);
         if (exp(sa_o[s] - sa_n[s]) >= rx) {
+          if (L->renorm_on_write) renorm_field(&h[s].a.re, 1);
           st_su2(UPTR(L, L->u, mu, eo, s), h[s]);
           acc1 += 1.0;
         }

@@ -36,6 +36,9 @@ void oc_set_couplings(oc_lattice *L, double beta, double beta_a, double beta_2,
 void oc_set_gauge_fixing(oc_lattice *L, double lambda, double alpha,
                          double limit_or, int max_iter);
 void oc_set_imode(oc_lattice *L, int imode);
+/* production sweeps (lceq_sweep schedule 0) normalise every accepted link
+ * update as f/renorm.f does; 1 switches the link updates to that mode */
+void oc_set_renorm_on_write(oc_lattice *L, int flag);
 
 void oc_rng_feed(oc_lattice *L, const double *buf, int64_t n);
 int64_t oc_rng_consumed(const oc_lattice *L);

@@ -44,6 +44,7 @@ def lib():
             "oc_set_couplings": (None, [VP, D, D, D, D]),
             "oc_set_gauge_fixing": (None, [VP, D, D, D, I]),
             "oc_set_imode": (None, [VP, I]),
+            "oc_set_renorm_on_write": (None, [VP, I]),
             "oc_rng_feed": (None, [VP, DP, I64]),
             "oc_rng_consumed": (I64, [VP]),
             "oc_rng_philox": (None, [VP, U64, U64]),
@@ -173,6 +174,9 @@ class OracleLattice:
 
     def set_imode(self, imode):
         self._L.oc_set_imode(self.h, imode)
+
+    def set_renorm_on_write(self, flag):
+        self._L.oc_set_renorm_on_write(self.h, int(flag))
 
     def feed(self, uniforms):
         self._fed = np.ascontiguousarray(uniforms, dtype=np.float64)

@@ -90,24 +90,27 @@ def test_philox_higgs_updates(make, omake, shape, imode):
     assert_close(G.get_higgs(), O.get_higgs(), 1e-12, "higgs")
 
 
-def test_philox_production_sweeps_16(make, omake):
-    """Five sweeps of 1 KP + 4 OR + ahiggs1 at the production couplings."""
+@pytest.mark.parametrize("nsweep,tol", [(1, 1e-12), (2, 1e-12), (6, 1e-9)])
+def test_philox_production_sweeps_16(make, omake, nsweep, tol):
+    """1 KP + 4 OR + ahiggs1 sweeps at the production couplings, GPU ==
+    oracle.  Roundoff differences (FMA contraction, libm ulps) grow by roughly
+    an order of magnitude per sweep in this chaotic Markov chain, so the
+    tolerance after six sweeps is wider."""
     shape = (16, 16, 16)
     G, O = _pair(make, omake, shape, 3, coup=PROD, asig=2.0)
+    O.set_renorm_on_write(1)
     d = 0.4
-    for it in range(5):
+    for it in range(nsweep):
         O.agauge(0)
         for k in range(4):
             O.link_or(0)
         d, _ = O.ahiggs1(d, 1)
-    dG = G.sweep(0, 5, n_or=4, delta=0.4, measflag=1)
+    dG = G.sweep(0, nsweep, n_or=4, delta=0.4, measflag=1)
     assert dG == d
-    # the device sweep renormalises on write; compare after renormalising the oracle
-    O.renorm()
-    assert_close(G.get_links(), O.get_links(), 1e-11, "links after 5 sweeps")
-    assert_close(G.get_higgs(), O.get_higgs(), 1e-11, "higgs after 5 sweeps")
-    h = G.history(5)
-    assert_close(h[-1, 2], O.observables()["a02"], 1e-11, "a02")
+    assert_close(G.get_links(), O.get_links(), tol, "links")
+    assert_close(G.get_higgs(), O.get_higgs(), tol, "higgs")
+    h = G.history(nsweep)
+    assert_close(h[-1, 2], O.observables()["a02"], tol, "a02")
 
 
 def test_reference_schedule_sweeps(make, omake):
