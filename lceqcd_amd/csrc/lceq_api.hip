@@ -91,6 +91,9 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   alloc(&L->Vst, 24 * np);
   alloc(&L->F, 3 * np);
   alloc(&L->partials, (size_t)LCEQ_NSLOT * LCEQ_NQ * L->pstride);
+  alloc(&L->slot_tot, (size_t)LCEQ_NSLOT * LCEQ_NQ);
+  if (rc == LCEQ_OK && hipMalloc((void**)&L->counters, LCEQ_NSLOT * sizeof(unsigned)) != hipSuccess)
+    rc = fail(LCEQ_ERR_HIP, "hipMalloc(counters) failed");
   alloc(&L->obs, LCEQ_OBS_N);
   alloc(&L->slices, 14 * (size_t)lz);
   alloc(&L->corr, 5 * (size_t)lz + 8);
@@ -107,6 +110,8 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
     (void)hipMemsetAsync(L->partials, 0, (size_t)LCEQ_NSLOT * LCEQ_NQ * L->pstride * sizeof(double),
                    L->stream);
     (void)hipMemsetAsync(L->obs, 0, LCEQ_OBS_N * sizeof(double), L->stream);
+    (void)hipMemsetAsync(L->slot_tot, 0, (size_t)LCEQ_NSLOT * LCEQ_NQ * sizeof(double), L->stream);
+    (void)hipMemsetAsync(L->counters, 0, LCEQ_NSLOT * sizeof(unsigned), L->stream);
     if (hipStreamSynchronize(L->stream) != hipSuccess) rc = fail(LCEQ_ERR_HIP, "init sync failed");
   }
   if (rc != LCEQ_OK) {
@@ -120,8 +125,9 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
 extern "C" int lceq_destroy(lceq_lattice* L) {
   if (!L) return LCEQ_OK;
   if (L->stream) (void)hipStreamSynchronize(L->stream);
-  double* bufs[] = {L->U, L->A, L->Uf, L->Af, L->Vst, L->F, L->partials, L->obs,
+  double* bufs[] = {L->U, L->A, L->Uf, L->Af, L->Vst, L->F, L->partials, L->slot_tot, L->obs,
                     L->slices, L->corr, L->hist, L->scratch, L->fed};
+  if (L->counters) (void)hipFree(L->counters);
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (L->prof_ev) {
@@ -381,8 +387,11 @@ extern "C" int lceq_heat_kpi(lceq_lattice* L, int n, const double* b, double* v,
   return LCEQ_OK;
 }
 
-// one full link sweep (eo outer, mu inner as in f/agauge.f:53-57)
-static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write) {
+// one full link sweep (eo outer, mu inner as in f/agauge.f:53-57); with
+// reduce the six launches accumulate the plaquette and acceptance and the
+// last one writes the observables (and the history row, if given)
+static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce,
+                      double* hist_row) {
   int64_t per = (kp ? 7 : 1) * (int64_t)L->g.n;
   const double* fed0;
   int rc = take_uniforms(L, 6 * per, &fed0);
@@ -390,18 +399,17 @@ static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write
   for (int eo = 0; eo < 2; ++eo)
     for (int mu = 0; mu < 3; ++mu) {
       int slot = eo * 3 + mu;
+      int last = slot == 5;
       launch_link(L, kp, eo, mu, measflag, renorm_on_write, fed0 ? fed0 + slot * per : nullptr,
-                  next_pass(L), slot);
+                  next_pass(L), slot, reduce, reduce && last, last ? hist_row : nullptr);
     }
   return check_launch();
 }
 
-extern "C" int lceq_agauge(lceq_lattice* L, int measflag, double* accept) {
+static int link_call(lceq_lattice* L, int kp, int measflag, double* accept) {
   NEEDL(L);
-  int rc = link_sweep(L, 1, measflag, 0);
+  int rc = link_sweep(L, kp, measflag, 0, 1, nullptr);
   if (rc) return rc;
-  launch_link_finalize(L, 1, measflag);
-  if ((rc = check_launch())) return rc;
   if (accept) {
     double o[LCEQ_OBS_N];
     if ((rc = read_obs(L, o))) return rc;
@@ -410,18 +418,12 @@ extern "C" int lceq_agauge(lceq_lattice* L, int measflag, double* accept) {
   return LCEQ_OK;
 }
 
+extern "C" int lceq_agauge(lceq_lattice* L, int measflag, double* accept) {
+  return link_call(L, 1, measflag, accept);
+}
+
 extern "C" int lceq_link_or(lceq_lattice* L, int measflag, double* accept) {
-  NEEDL(L);
-  int rc = link_sweep(L, 0, measflag, 0);
-  if (rc) return rc;
-  launch_link_finalize(L, 0, measflag);
-  if ((rc = check_launch())) return rc;
-  if (accept) {
-    double o[LCEQ_OBS_N];
-    if ((rc = read_obs(L, o))) return rc;
-    *accept = o[OBS_ACC_LINK];
-  }
-  return LCEQ_OK;
+  return link_call(L, 0, measflag, accept);
 }
 
 extern "C" int lceq_a_force(lceq_lattice* L, int eo) {
@@ -448,14 +450,14 @@ extern "C" int lceq_gauss(lceq_lattice* L, double* x, const double* r, int nh) {
 }
 
 // both parities of one Higgs routine (kind 1/2/3 = ahiggs1/2/3)
-static int higgs_sweep(lceq_lattice* L, int kind, int measflag) {
+static int higgs_sweep(lceq_lattice* L, int kind, int measflag, double* hist_row) {
   int64_t per = (kind == 1 ? 4 : kind == 2 ? 7 : 1) * (int64_t)L->g.n;
   const double* fed0;
   int rc = take_uniforms(L, 2 * per, &fed0);
   if (rc) return rc;
   for (int eo = 0; eo < 2; ++eo)
-    launch_higgs(L, kind, eo, measflag, fed0 ? fed0 + eo * per : nullptr, next_pass(L), eo);
-  launch_higgs_finalize(L, kind, measflag);
+    launch_higgs(L, kind, eo, measflag, fed0 ? fed0 + eo * per : nullptr, next_pass(L), eo, eo == 1,
+                 eo == 1 ? hist_row : nullptr);
   return check_launch();
 }
 
@@ -464,7 +466,7 @@ extern "C" int lceq_ahiggs1(lceq_lattice* L, double* delta, int measflag, double
   if (!delta) return fail(LCEQ_ERR_ARG, "delta is required");
   int rc = write_obs(L, OBS_DELTA, *delta);
   if (rc) return rc;
-  if ((rc = higgs_sweep(L, 1, measflag))) return rc;
+  if ((rc = higgs_sweep(L, 1, measflag, nullptr))) return rc;
   double o[LCEQ_OBS_N];
   if ((rc = read_obs(L, o))) return rc;
   *delta = o[OBS_DELTA];
@@ -474,7 +476,7 @@ extern "C" int lceq_ahiggs1(lceq_lattice* L, double* delta, int measflag, double
 
 static int higgs_call(lceq_lattice* L, int kind, int measflag, double* accept) {
   NEEDL(L);
-  int rc = higgs_sweep(L, kind, measflag);
+  int rc = higgs_sweep(L, kind, measflag, nullptr);
   if (rc) return rc;
   if (accept && measflag == 1) {
     double o[LCEQ_OBS_N];
@@ -505,7 +507,6 @@ extern "C" int lceq_or_gauge_fixing(lceq_lattice* L, double* da1) {
   NEEDL(L);
   launch_gfix(L, 0, 0, 0, 0, 0);
   launch_gfix(L, 1, 1, 0, 0, 0);
-  launch_gfix_finalize(L, 0);
   int rc = check_launch();
   if (rc) return rc;
   if (da1) {
@@ -534,7 +535,6 @@ extern "C" int lceq_landau_gauge(lceq_lattice* L, double* da1, int* iter_or1) {
     for (int k = 0; k < batch; ++k) {
       launch_gfix(L, 0, 0, 0, 1, 0);
       launch_gfix(L, 1, 1, 1, 1, 0);
-      launch_gfix_finalize(L, 1);
     }
     int rc = check_launch();
     if (rc) return rc;
@@ -657,28 +657,25 @@ extern "C" int lceq_sweep(lceq_lattice* L, int schedule, int nsweep, int n_or, d
     if (!delta) return fail(LCEQ_ERR_ARG, "delta is required for schedule 0");
     if ((rc = write_obs(L, OBS_DELTA, *delta))) return rc;
   }
+  if (measflag && L->hist_rows + nsweep > L->hist_cap)
+    return fail(LCEQ_ERR_ARG, "history full (%d rows); call lceq_history", L->hist_cap);
   for (int it = 0; it < nsweep; ++it) {
+    double* row = measflag ? L->hist + (size_t)(L->hist_rows++) * LCEQ_OBS_N : nullptr;
     if (schedule == LCEQ_SWEEP_KP_OR_METRO) {
-      // the last link update of the sweep carries the measurement
+      // only the last link sweep carries the measurement; the ahiggs1 step
+      // adapts every sweep, so the Higgs launches always reduce
       int meas_kp = measflag && n_or == 0;
-      if ((rc = link_sweep(L, 1, meas_kp, 1))) return rc;
-      if (meas_kp) launch_link_finalize(L, 1, 1);
+      if ((rc = link_sweep(L, 1, meas_kp, 1, meas_kp, nullptr))) return rc;
       for (int k = 0; k < n_or; ++k) {
         int m = measflag && k == n_or - 1;
-        if ((rc = link_sweep(L, 0, m, 1))) return rc;
-        if (m) launch_link_finalize(L, 0, 1);
+        if ((rc = link_sweep(L, 0, m, 1, m, nullptr))) return rc;
       }
-      if ((rc = higgs_sweep(L, 1, measflag))) return rc;
+      if ((rc = higgs_sweep(L, 1, measflag, row))) return rc;
     } else {  // f/main.f:38-41
-      if ((rc = higgs_sweep(L, 2, measflag))) return rc;
-      if ((rc = higgs_sweep(L, 3, measflag))) return rc;
-      if ((rc = link_sweep(L, 1, measflag, 0))) return rc;
-      launch_link_finalize(L, 1, measflag);
+      if ((rc = higgs_sweep(L, 2, measflag, nullptr))) return rc;
+      if ((rc = higgs_sweep(L, 3, measflag, nullptr))) return rc;
+      if ((rc = link_sweep(L, 1, measflag, 0, 1, row))) return rc;
       launch_renorm(L, L->U);
-    }
-    if (measflag) {
-      if (L->hist_rows >= L->hist_cap) return fail(LCEQ_ERR_ARG, "history full; call lceq_history");
-      launch_record(L, L->hist_rows++);
     }
   }
   if ((rc = check_launch())) return rc;

@@ -82,12 +82,12 @@ __device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
   int xh = r - y * g.h1;
   return Coord{2 * xh + ((p + y + z) & 1), y, z};
 }
-__device__ __forceinline__ int site_index(const Geom& g, int x, int y, int z) {
-  return (x >> 1) + g.h1 * y + g.h2 * z;
+__device__ __forceinline__ unsigned site_index(const Geom& g, int x, int y, int z) {
+  return (unsigned)((x >> 1) + g.h1 * y + g.h2 * z);
 }
 __device__ __forceinline__ int wrap(int c, int l) { return c < 0 ? c + l : (c >= l ? c - l : c); }
 // index of (c + d), d a displacement with components in {-1,0,1}
-__device__ __forceinline__ int shifted(const Geom& g, Coord c, int dx, int dy, int dz) {
+__device__ __forceinline__ unsigned shifted(const Geom& g, Coord c, int dx, int dy, int dz) {
   return site_index(g, wrap(c.x + dx, g.lx), wrap(c.y + dy, g.ly), wrap(c.z + dz, g.lz));
 }
 __device__ __forceinline__ void unit(int mu, int& dx, int& dy, int& dz) {
@@ -101,30 +101,45 @@ __device__ __forceinline__ void unit(int mu, int& dx, int& dy, int& dz) {
 // Links: component c of U_mu at parity p, site s at
 //   U[((mu*2 + p)*4 + c)*npad + s],   c = Re alpha, Im alpha, Re beta, Im beta.
 // Higgs: A[(p*3 + i)*npad + s].
+// Component bases are uniform per launch and the site offset is an unsigned
+// 32-bit byte offset, so the loads can use the SGPR-base + 32-bit VGPR-offset
+// form (arrays stay below 4 GiB per component).
+template <typename T>
+__device__ __forceinline__ T& at(T* base, unsigned s) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (s << 3));
+}
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, unsigned s) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (s << 3));
+}
 __device__ __forceinline__ su2 ld_link(const double* __restrict__ U, const Geom& g, int mu,
-                                       int p, int s) {
-  const double* q = U + (size_t)((mu * 2 + p) * 4) * g.npad + s;
-  return su2{c_mk(q[0], q[g.npad]), c_mk(q[2 * (size_t)g.npad], q[3 * (size_t)g.npad])};
+                                       int p, unsigned s) {
+  const double* q = U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad;
+  const size_t np = (size_t)g.npad;
+  return su2{c_mk(at(q, s), at(q + np, s)), c_mk(at(q + 2 * np, s), at(q + 3 * np, s))};
 }
 __device__ __forceinline__ void st_link(double* __restrict__ U, const Geom& g, int mu, int p,
-                                        int s, su2 u) {
-  double* q = U + (size_t)((mu * 2 + p) * 4) * g.npad + s;
-  q[0] = u.a.re;
-  q[g.npad] = u.a.im;
-  q[2 * (size_t)g.npad] = u.b.re;
-  q[3 * (size_t)g.npad] = u.b.im;
+                                        unsigned s, su2 u) {
+  double* q = U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad;
+  const size_t np = (size_t)g.npad;
+  at(q, s) = u.a.re;
+  at(q + np, s) = u.a.im;
+  at(q + 2 * np, s) = u.b.re;
+  at(q + 3 * np, s) = u.b.im;
 }
 __device__ __forceinline__ vec3 ld_higgs(const double* __restrict__ A, const Geom& g, int p,
-                                         int s) {
-  const double* q = A + (size_t)(p * 3) * g.npad + s;
-  return vec3{{q[0], q[g.npad], q[2 * (size_t)g.npad]}};
+                                         unsigned s) {
+  const double* q = A + (size_t)(p * 3) * (size_t)g.npad;
+  const size_t np = (size_t)g.npad;
+  return vec3{{at(q, s), at(q + np, s), at(q + 2 * np, s)}};
 }
-__device__ __forceinline__ void st_higgs(double* __restrict__ A, const Geom& g, int p, int s,
+__device__ __forceinline__ void st_higgs(double* __restrict__ A, const Geom& g, int p, unsigned s,
                                          const vec3& a) {
-  double* q = A + (size_t)(p * 3) * g.npad + s;
-  q[0] = a.x[0];
-  q[g.npad] = a.x[1];
-  q[2 * (size_t)g.npad] = a.x[2];
+  double* q = A + (size_t)(p * 3) * (size_t)g.npad;
+  const size_t np = (size_t)g.npad;
+  at(q, s) = a.x[0];
+  at(q + np, s) = a.x[1];
+  at(q + 2 * np, s) = a.x[2];
 }
 
 // ----------------------------------------------------------------- RNG
@@ -172,12 +187,12 @@ __device__ __forceinline__ void philox_pair(const Philox& key, uint32_t idx, uin
 
 // ------------------------------------------------------------ reductions
 
-// Deterministic block sum of NQ doubles (blockDim = 256 = 4 waves of 64):
+// Deterministic in-block sum of NQ doubles (blockDim = 256 = 4 waves of 64):
 // xor-butterfly inside each wave, then the four wave totals in fixed order.
+// Every thread of the block receives the NQ totals.
 template <int NQ>
-__device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restrict__ out,
-                                                int stride, int blk) {
-  __shared__ double red[4][NQ > 0 ? NQ : 1];
+__device__ __forceinline__ void block_sum(double (&v)[NQ]) {
+  __shared__ double red[4][NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     double t = v[q];
@@ -191,10 +206,61 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restr
     for (int q = 0; q < NQ; ++q) red[wv][q] = v[q];
   }
   __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  __syncthreads();
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restrict__ out,
+                                                int stride, int blk) {
+  block_sum<NQ>(v);
   if (threadIdx.x < NQ) {
     int q = threadIdx.x;
-    out[(size_t)q * stride + blk] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    double t = v[0];
+#pragma unroll
+    for (int k = 1; k < NQ; ++k)
+      if (q == k) t = v[k];
+    out[(size_t)q * stride + blk] = t;
   }
+}
+
+// One launch's grid-wide reduction slot: per-block partials, an arrival
+// counter (zero between launches) and the slot totals.
+struct Reduce {
+  double* part;       // [NQ][pstride]
+  unsigned* counter;
+  double* tot;        // [NQ]
+  int pstride, nb;
+};
+
+// Store this block's partials; the block that arrives last sums all nb
+// partials of each quantity in a fixed order (so the result does not depend
+// on arrival order), stores the totals, re-arms the counter and returns true
+// with the totals in v.  Replaces a separate reduction launch.
+template <int NQ>
+__device__ __forceinline__ bool block_reduce_last(double (&v)[NQ], const Reduce& r, int blk) {
+  __shared__ int is_last;
+  block_sum_store<NQ>(v, r.part, r.pstride, blk);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = (atomicAdd(r.counter, 1u) == (unsigned)(r.nb - 1));
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = 0.0;
+  for (int i = threadIdx.x; i < r.nb; i += 256) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] += r.part[(size_t)q * r.pstride + i];
+  }
+  block_sum<NQ>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) r.tot[q] = v[q];
+    *r.counter = 0u;
+  }
+  return true;
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; hand each XCD a contiguous

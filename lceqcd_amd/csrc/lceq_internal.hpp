@@ -39,6 +39,8 @@ struct lceq_lattice {
   double *U, *A, *Uf, *Af, *Vst, *F;
   double *partials;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
   int pstride;       // >= number of 256-site blocks
+  unsigned *counters;  // [LCEQ_NSLOT] arrival counters of the last-block reductions
+  double *slot_tot;  // [LCEQ_NSLOT][LCEQ_NQ] per-launch totals
   double *obs;       // [LCEQ_OBS_N]
   double *hist;      // [hist_cap][LCEQ_OBS_N] per-sweep records
   int hist_cap;
@@ -67,18 +69,19 @@ enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC
 
 int blocks_for(int n);
 
-// launchers (lceq_kernels.hip); all enqueue on L->stream, no host sync
+// launchers (lceq_kernels.hip); all enqueue on L->stream, no host sync.
+// reduce: accumulate the launch's partial sums into slot `slot`; final: this
+// launch closes its routine and writes the observables (and, with hist_row,
+// one history row).
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
-                 const double* fed, uint64_t pass, int slot);
-void launch_link_finalize(lceq_lattice* L, int kp, int measflag);
+                 const double* fed, uint64_t pass, int slot, int reduce, int final_,
+                 double* hist_row);
 void launch_staples(lceq_lattice* L, int eo, int mu);
 void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
-                  uint64_t pass, int slot);
-void launch_higgs_finalize(lceq_lattice* L, int kind, int measflag);
+                  uint64_t pass, int slot, int final_, double* hist_row);
 void launch_renorm(lceq_lattice* L, double* U);
 void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int slot);
-void launch_gfix_finalize(lceq_lattice* L, int guarded);
 void launch_slices(lceq_lattice* L);
 void launch_corr(lceq_lattice* L);
 void launch_heat_kpi(lceq_lattice* L, int n, const double* b, const double* a0_old, double* v,
@@ -90,6 +93,5 @@ void launch_multall(lceq_lattice* L, int kind, const double* op1, const double* 
 void launch_hotstart(lceq_lattice* L, uint64_t seed, double sigma);
 void launch_unitconf(lceq_lattice* L);
 void launch_testconf(lceq_lattice* L);
-void launch_record(lceq_lattice* L, int index);
 
 }  // namespace lceq

@@ -48,34 +48,98 @@ struct LinkArgs {
   Philox key;
   uint64_t pass;
   int measflag, renorm;
-  double* part;  // partials of this slot: [LCEQ_NQ][pstride]
-  int pstride, nb;
+  int reduce, final_, slot;
+  Reduce red;
+  const double* slot_tot;  // [LCEQ_NSLOT][LCEQ_NQ] totals of this routine's earlier launches
+  double vol;
+  double* obs;
+  double* hist_row;        // copy the observables here when final_ (may be null)
+  int nb;
 };
 
-// staple V(x) of f/staples.f:65-216 for x in parity eo, direction mu
-__device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& g, int eo, int mu,
-                                      int s, Coord c) {
+// Observables of a complete link sweep from the six (eo, mu) slot totals in
+// reference order (f/agauge.f:195-200); slot 5's totals are in last[].
+__device__ void link_observables(const LinkArgs& p, const double (&last)[3]) {
+  double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
+  for (int slot = 0; slot < 6; ++slot) {
+    const double* t = slot == 5 ? last : p.slot_tot + slot * LCEQ_NQ;
+    plaq = plaq + t[0];
+    nacch += t[1];
+    acc1 += t[2];
+  }
+  double* obs = p.obs;
+  obs[OBS_NACCH] = nacch;
+  obs[OBS_NACC1] = acc1;
+  if (p.measflag == 1) obs[OBS_PLAQ0] = plaq / (12.0 * p.vol);
+  double accept;
+  if (p.final_ == 1) {  // Kennedy-Pendleton: (nacch/3V)(acc1/12V) with measflag, else nacch
+    accept = nacch;
+    if (p.measflag == 1) accept = (accept / (3.0 * p.vol)) * (acc1 / (12.0 * p.vol));
+  } else {              // overrelaxation: Metropolis acceptance
+    accept = acc1 / (3.0 * p.vol);
+  }
+  obs[OBS_ACC_LINK] = accept;
+  if (p.hist_row)
+    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
+}
+
+// The 12 links of the staple V(x) of f/staples.f:65-216 for x in parity eo,
+// direction mu, two nu per site: t = 0, 1 for nu = the two directions != mu
+// in increasing order (f/staples.f:48-52).
+struct StapleLinks {
+  su2 mu_xpnu[2];    // U_mu(x+nu)      parity oe  (link 3)
+  su2 nu_x[2];       // U_nu(x)         parity eo  (link 2)
+  su2 nu_xpmu[2];    // U_nu(x+mu)      parity oe  (link 4)
+  su2 mu_xmnu[2];    // U_mu(x-nu)      parity oe  (link 6)
+  su2 nu_xmnu[2];    // U_nu(x-nu)      parity oe  (link 5)
+  su2 nu_xpmumnu[2]; // U_nu(x+mu-nu)   parity eo  (link 7)
+};
+
+__device__ __forceinline__ void load_staple_links(const double* __restrict__ U, const Geom& g,
+                                                  int eo, int mu, unsigned s, Coord c,
+                                                  unsigned sxm, StapleLinks& k) {
   int oe = eo ^ 1;
   int mx, my, mz;
   unit(mu, mx, my, mz);
-  int sxm = shifted(g, c, mx, my, mz);  // x+mu, parity oe
-  su2 V{c_mk(0.0, 0.0), c_mk(0.0, 0.0)};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     int nu = (t == 0) ? (mu == 0 ? 1 : 0) : (mu == 2 ? 1 : 2);
     int nx, ny, nz;
     unit(nu, nx, ny, nz);
-    int sxn = shifted(g, c, nx, ny, nz);                      // x+nu   (oe)
-    int sxmn = shifted(g, c, -nx, -ny, -nz);                  // x-nu   (oe)
-    int sxmmn = shifted(g, c, mx - nx, my - ny, mz - nz);     // x+mu-nu (eo)
-    su2 up1 = m_dd(ld_link(U, g, mu, oe, sxn), ld_link(U, g, nu, eo, s));     // 3^+ 2^+
-    su2 up2 = m_nn(ld_link(U, g, nu, oe, sxm), up1);                          // 4 3^+ 2^+
-    su2 bt1 = m_dn(ld_link(U, g, mu, oe, sxmn), ld_link(U, g, nu, oe, sxmn)); // 6^+ 5
-    su2 bt2 = m_dn(ld_link(U, g, nu, eo, sxmmn), bt1);                        // 7^+ 6^+ 5
+    unsigned sxn = shifted(g, c, nx, ny, nz);
+    unsigned sxmn = shifted(g, c, -nx, -ny, -nz);
+    unsigned sxmmn = shifted(g, c, mx - nx, my - ny, mz - nz);
+    k.mu_xpnu[t] = ld_link(U, g, mu, oe, sxn);
+    k.nu_x[t] = ld_link(U, g, nu, eo, s);
+    k.nu_xpmu[t] = ld_link(U, g, nu, oe, sxm);
+    k.mu_xmnu[t] = ld_link(U, g, mu, oe, sxmn);
+    k.nu_xmnu[t] = ld_link(U, g, nu, oe, sxmn);
+    k.nu_xpmumnu[t] = ld_link(U, g, nu, eo, sxmmn);
+  }
+}
+
+// V = sum_t [ 4 3^+ 2^+ + 7^+ 6^+ 5 ] in the accumulation order of f/staples.f:210-216
+__device__ __forceinline__ su2 staple_sum(const StapleLinks& k) {
+  su2 V{c_mk(0.0, 0.0), c_mk(0.0, 0.0)};
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    su2 up1 = m_dd(k.mu_xpnu[t], k.nu_x[t]);
+    su2 up2 = m_nn(k.nu_xpmu[t], up1);
+    su2 bt1 = m_dn(k.mu_xmnu[t], k.nu_xmnu[t]);
+    su2 bt2 = m_dn(k.nu_xpmumnu[t], bt1);
     V.a = c_add(c_add(V.a, up2.a), bt2.a);
     V.b = c_add(c_add(V.b, up2.b), bt2.b);
   }
   return V;
+}
+
+__device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& g, int eo, int mu,
+                                      unsigned s, Coord c) {
+  int mx, my, mz;
+  unit(mu, mx, my, mz);
+  StapleLinks k;
+  load_staple_links(U, g, eo, mu, s, c, shifted(g, c, mx, my, mz), k);
+  return staple_sum(k);
 }
 
 __device__ __forceinline__ double hop_action(double beta_a, const vec3& ax, const vec3& axp,
@@ -92,34 +156,31 @@ __device__ __forceinline__ su2 normalized(su2 u) {
 // One (eo, mu) class of f/agauge.f:53-193 (KP=true), or the link
 // overrelaxation U -> Vhat^+ U^+ Vhat^+ with the same hopping Metropolis
 // (KP=false).  Partials: 0 plaquette, 1 KP accepts, 2 Metropolis accepts.
+// All 58 operand loads (and the fed uniforms) are issued before any
+// arithmetic so that one memory latency covers the whole site; the Philox
+// rounds run while the loads are in flight.
 template <bool KP, bool FED>
 __global__ __launch_bounds__(256) void k_link(LinkArgs p) {
   const Geom& g = p.g;
   int blk = xcd_block(p.nb);
-  int s = blk * TB + threadIdx.x;
+  unsigned s = (unsigned)(blk * TB + threadIdx.x);
   double red[3] = {0.0, 0.0, 0.0};
-  if (s < g.n) {
+  if (s < (unsigned)g.n) {
     int eo = p.eo, oe = eo ^ 1, mu = p.mu;
-    Coord c = site_coord(g, eo, s);
+    Coord c = site_coord(g, eo, (int)s);
     int mx, my, mz;
     unit(mu, mx, my, mz);
-    int sxm = shifted(g, c, mx, my, mz);
-    su2 V = staple(p.U, g, eo, mu, s, c);
+    unsigned sxm = shifted(g, c, mx, my, mz);
+    // ---- load phase
+    StapleLinks k;
+    load_staple_links(p.U, g, eo, mu, s, c, sxm, k);
     su2 u = ld_link(p.U, g, mu, eo, s);
     vec3 ax = ld_higgs(p.A, g, eo, s), axp = ld_higgs(p.A, g, oe, sxm);
-    // f/agauge.f:71-84
-    double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
-    t = 1.0 / sqrt(t);
-    su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
-    su2 h;
-    double racc;
-    if (KP) {
-      double b = t / p.beta;
-      double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
-      double r1, r2, r3, r4, r5, r6;
-      if (FED) {
-        const double* f = p.fed;
-        int n = g.n;
+    double r1, r2, r3, r4, r5, r6, racc;
+    if (FED) {
+      const double* f = p.fed;
+      unsigned n = (unsigned)g.n;
+      if (KP) {
         r1 = f[3 * s];
         r2 = f[3 * s + 1];
         r3 = f[3 * s + 2];
@@ -128,12 +189,31 @@ __global__ __launch_bounds__(256) void k_link(LinkArgs p) {
         r6 = f[5 * n + s];
         racc = f[6 * n + s];
       } else {
-        double dummy;
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, r1, r2);
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, r3, r4);
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 2, r5, r6);
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 3, racc, dummy);
+        racc = f[s];
       }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (!FED) {
+      double dummy;
+      if (KP) {
+        philox_pair(p.key, s, 0, p.pass, 0, r1, r2);
+        philox_pair(p.key, s, 0, p.pass, 1, r3, r4);
+        philox_pair(p.key, s, 0, p.pass, 2, r5, r6);
+        philox_pair(p.key, s, 0, p.pass, 3, racc, dummy);
+      } else {
+        philox_pair(p.key, s, 0, p.pass, 0, racc, dummy);
+      }
+    }
+    // ---- compute
+    su2 V = staple_sum(k);
+    // f/agauge.f:71-84
+    double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
+    t = 1.0 / sqrt(t);
+    su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
+    su2 h;
+    if (KP) {
+      double b = t / p.beta;
+      double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
       // Kennedy-Pendleton, f/heat_kpi.f:42-98
       if (r1 == 0.0) r1 = 0.0000033;
       if (r2 == 0.0) r2 = 0.0000066;
@@ -153,12 +233,6 @@ __global__ __launch_bounds__(256) void k_link(LinkArgs p) {
       w.b = c_mk(a0b * sin(ang), a0b * cos(ang));
       h = m_nd(w, vh);  // f/agauge.f:98
     } else {
-      if (FED) {
-        racc = p.fed[s];
-      } else {
-        double dummy;
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, racc, dummy);
-      }
       su2 r = m_nn(m_nn(vh, u), vh);
       h = su2{c_conj(r.a), c_neg(r.b)};
     }
@@ -171,43 +245,23 @@ __global__ __launch_bounds__(256) void k_link(LinkArgs p) {
     }
     if (p.measflag == 1) red[0] = m_nn(u, V).a.re;  // f/agauge.f:179-185
   }
-  block_sum_store<3>(red, p.part, p.pstride, blk);
+  if (p.reduce && block_reduce_last<3>(red, p.red, blk) && p.final_ && threadIdx.x == 0)
+    link_observables(p, red);
 }
 
-// Combine the six (eo, mu) slots in reference order (f/agauge.f:195-200).
-__global__ __launch_bounds__(256) void k_link_finalize(const double* __restrict__ part, int pstride,
-                                                       int nb, double vol, int kp, int measflag,
-                                                       double* __restrict__ obs) {
-  double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
-  for (int slot = 0; slot < 6; ++slot) {
-    const double* ps = part + (size_t)slot * LCEQ_NQ * pstride;
-    double e = block_sum_range(ps, nb);
-    double k = block_sum_range(ps + pstride, nb);
-    double m = block_sum_range(ps + 2 * (size_t)pstride, nb);
-    plaq = plaq + e;
-    nacch += k;
-    acc1 += m;
-  }
-  if (threadIdx.x == 0) {
-    obs[OBS_NACCH] = nacch;
-    obs[OBS_NACC1] = acc1;
-    if (kp) {
-      double accept = nacch;
-      if (measflag == 1) {
-        obs[OBS_PLAQ0] = plaq / (12.0 * vol);
-        accept = accept / (3.0 * vol);
-        accept = accept * (acc1 / (12.0 * vol));
-      }
-      obs[OBS_ACC_LINK] = accept;
-    } else {
-      if (measflag == 1) obs[OBS_PLAQ0] = plaq / (12.0 * vol);
-      obs[OBS_ACC_LINK] = acc1 / (3.0 * vol);
-    }
-  }
+static Reduce slot_reduce(lceq_lattice* L, int slot) {
+  Reduce r;
+  r.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
+  r.counter = L->counters + slot;
+  r.tot = L->slot_tot + slot * LCEQ_NQ;
+  r.pstride = L->pstride;
+  r.nb = blocks_for(L->g.n);
+  return r;
 }
 
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
-                 const double* fed, uint64_t pass, int slot) {
+                 const double* fed, uint64_t pass, int slot, int reduce, int final_,
+                 double* hist_row) {
   LinkArgs a;
   a.g = L->g;
   a.U = L->U;
@@ -221,8 +275,14 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.pass = pass;
   a.measflag = measflag;
   a.renorm = renorm_on_write;
-  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
-  a.pstride = L->pstride;
+  a.reduce = reduce;
+  a.final_ = final_ ? (kp ? 1 : 2) : 0;
+  a.slot = slot;
+  a.red = slot_reduce(L, slot);
+  a.slot_tot = L->slot_tot;
+  a.vol = (double)L->vol;
+  a.obs = L->obs;
+  a.hist_row = hist_row;
   a.nb = blocks_for(L->g.n);
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
@@ -238,11 +298,6 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
       hipLaunchKernelGGL((k_link<false, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
   }
   prof_end(L, kc);
-}
-
-void launch_link_finalize(lceq_lattice* L, int kp, int measflag) {
-  hipLaunchKernelGGL(k_link_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials, L->pstride,
-                     blocks_for(L->g.n), (double)L->vol, kp, measflag, L->obs);
 }
 
 // staple only, for the staples() entry point (f/staples.f:9)
@@ -261,26 +316,50 @@ void launch_staples(lceq_lattice* L, int eo, int mu) {
 
 // ================================================================ Higgs
 
-// F(x) of f/a_force.f:54-110 for x in parity eo
-__device__ __forceinline__ vec3 hop_force(const double* __restrict__ U,
-                                          const double* __restrict__ A, const Geom& g, int eo,
-                                          int s, Coord c) {
+// Operands of the hopping force F(x) of f/a_force.f:54-110 for x in parity eo
+struct ForceLinks {
+  su2 ux[3];   // U_mu(x)        parity eo
+  su2 uy[3];   // U_mu(x-mu)     parity oe
+  vec3 axp[3]; // a(x+mu)        parity oe
+  vec3 axd[3]; // a(x-mu)        parity oe
+};
+
+__device__ __forceinline__ void load_force_links(const double* __restrict__ U,
+                                                 const double* __restrict__ A, const Geom& g,
+                                                 int eo, unsigned s, Coord c, ForceLinks& k) {
   int oe = eo ^ 1;
-  vec3 F{{0.0, 0.0, 0.0}};
 #pragma unroll
   for (int mu = 0; mu < 3; ++mu) {
     int mx, my, mz;
     unit(mu, mx, my, mz);
-    int sxp = shifted(g, c, mx, my, mz), sxd = shifted(g, c, -mx, -my, -mz);
-    su2 ux = ld_link(U, g, mu, eo, s);
-    su2 t1 = m_nn(ux, m_ad(ld_higgs(A, g, oe, sxp), ux));
-    su2 uy = ld_link(U, g, mu, oe, sxd);
-    su2 t2 = m_dn(uy, m_an(ld_higgs(A, g, oe, sxd), uy));
+    unsigned sxp = shifted(g, c, mx, my, mz), sxd = shifted(g, c, -mx, -my, -mz);
+    k.ux[mu] = ld_link(U, g, mu, eo, s);
+    k.uy[mu] = ld_link(U, g, mu, oe, sxd);
+    k.axp[mu] = ld_higgs(A, g, oe, sxp);
+    k.axd[mu] = ld_higgs(A, g, oe, sxd);
+  }
+}
+
+// F = sum_mu extract[U(x) X(a(x+mu)) U(x)^+ + U(x-mu)^+ X(a(x-mu)) U(x-mu)]
+__device__ __forceinline__ vec3 force_sum(const ForceLinks& k) {
+  vec3 F{{0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    su2 t1 = m_nn(k.ux[mu], m_ad(k.axp[mu], k.ux[mu]));
+    su2 t2 = m_dn(k.uy[mu], m_an(k.axd[mu], k.uy[mu]));
     F.x[2] = F.x[2] + c_add(t1.a, t2.a).im;
     F.x[1] = F.x[1] + c_add(t1.b, t2.b).re;
     F.x[0] = F.x[0] + c_add(t1.b, t2.b).im;
   }
   return F;
+}
+
+__device__ __forceinline__ vec3 hop_force(const double* __restrict__ U,
+                                          const double* __restrict__ A, const Geom& g, int eo,
+                                          unsigned s, Coord c) {
+  ForceLinks k;
+  load_force_links(U, A, g, eo, s, c, k);
+  return force_sum(k);
 }
 
 __global__ __launch_bounds__(256) void k_force(Geom g, const double* __restrict__ U,
@@ -309,10 +388,38 @@ struct HiggsArgs {
   const double* fed;
   Philox key;
   uint64_t pass;
-  int measflag;
-  double* part;
-  int pstride, nb;
+  int measflag, kind, final_;
+  Reduce red;
+  const double* slot0_tot;  // totals of the eo = 0 launch
+  double vol;
+  double* obs;
+  double* hist_row;
+  int nb;
 };
+
+// f/ahiggs1.f:107-123, f/ahiggs2.f:138-143 from the two parity totals
+__device__ void higgs_observables(const HiggsArgs& p, const double (&last)[4]) {
+  const double* t0 = p.slot0_tot;
+  double acc = t0[0] + last[0];
+  double r2 = t0[1] + last[1], r4 = t0[2] + last[2], kin = t0[3] + last[3];
+  double* obs = p.obs;
+  if (p.measflag == 1) {
+    obs[OBS_A02] = r2 / p.vol;
+    obs[OBS_A04] = r4 / p.vol;
+    obs[OBS_A0KIN] = kin / (6.0 * p.vol);
+  }
+  double accept = acc / p.vol;
+  if (p.kind == 1) {
+    obs[OBS_ACC_HIGGS] = accept;
+    double d = obs[OBS_DELTA];
+    if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+    obs[OBS_DELTA] = d;
+  } else if (p.measflag == 1) {
+    obs[OBS_ACC_HIGGS] = accept;
+  }
+  if (p.hist_row)
+    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
+}
 
 // KIND 1: f/ahiggs1.f Metropolis; 2: f/ahiggs2.f heatbath; 3: f/ahiggs3.f
 // overrelaxation.  Partials: 0 accepts, 1 rho, 2 rho^2, 3 a.F (post update).
@@ -320,21 +427,40 @@ template <int KIND, bool FED>
 __global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
   const Geom& g = p.g;
   int blk = xcd_block(p.nb);
-  int s = blk * TB + threadIdx.x;
+  unsigned s = (unsigned)(blk * TB + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  if (s < g.n) {
-    int eo = p.eo, n = g.n;
-    Coord c = site_coord(g, eo, s);
-    vec3 F = hop_force(p.U, p.A, g, eo, s, c);
+  if (s < (unsigned)g.n) {
+    int eo = p.eo;
+    unsigned n = (unsigned)g.n;
+    Coord c = site_coord(g, eo, (int)s);
+    // ---- load phase: 6 links, 7 Higgs triples and the fed uniforms
+    ForceLinks k;
+    load_force_links(p.U, p.A, g, eo, s, c, k);
     vec3 a = ld_higgs(p.A, g, eo, s), an;
+    double fu[7];
+    if (FED) {
+      if (KIND == 1) {
+        fu[0] = p.fed[3 * s];
+        fu[1] = p.fed[3 * s + 1];
+        fu[2] = p.fed[3 * s + 2];
+        fu[3] = p.fed[3 * n + s];
+      } else if (KIND == 2) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) fu[q] = p.fed[(size_t)q * n + s];
+      } else {
+        fu[0] = p.fed[s];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    vec3 F = force_sum(k);
     double ro = 0.0, rn = 0.0, boltz, racc;
     if (KIND == 1) {  // f/ahiggs1.f:51-92
       double delta = *p.delta, r[3];
       if (FED) {
-        r[0] = p.fed[3 * s];
-        r[1] = p.fed[3 * s + 1];
-        r[2] = p.fed[3 * s + 2];
-        racc = p.fed[3 * n + s];
+        r[0] = fu[0];
+        r[1] = fu[1];
+        r[2] = fu[2];
+        racc = fu[3];
       } else {
         philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, r[0], r[1]);
         double dummy;
@@ -371,8 +497,8 @@ __global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
         double w[6];
         if (FED) {
 #pragma unroll
-          for (int k = 0; k < 6; ++k) w[k] = p.fed[(size_t)k * n + s];
-          racc = p.fed[6 * (size_t)n + s];
+          for (int q = 0; q < 6; ++q) w[q] = fu[q];
+          racc = fu[6];
         } else {
           double dummy;
           philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, w[0], w[1]);
@@ -388,7 +514,7 @@ __global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
         }
       } else {  // f/ahiggs3.f:83-87
         if (FED) {
-          racc = p.fed[s];
+          racc = fu[0];
         } else {
           double dummy;
           philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, racc, dummy);
@@ -415,41 +541,12 @@ __global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
       red[3] = (a.x[0] * F.x[0] + a.x[1] * F.x[1]) + a.x[2] * F.x[2];
     }
   }
-  block_sum_store<4>(red, p.part, p.pstride, blk);
-}
-
-// f/ahiggs1.f:107-123, f/ahiggs2.f:310-315
-__global__ __launch_bounds__(256) void k_higgs_finalize(const double* __restrict__ part, int pstride,
-                                                        int nb, double vol, int kind, int measflag,
-                                                        double* __restrict__ obs) {
-  double acc = 0.0, r2 = 0.0, r4 = 0.0, kin = 0.0;
-  for (int slot = 0; slot < 2; ++slot) {
-    const double* ps = part + (size_t)slot * LCEQ_NQ * pstride;
-    acc += block_sum_range(ps, nb);
-    r2 = r2 + block_sum_range(ps + pstride, nb);
-    r4 = r4 + block_sum_range(ps + 2 * (size_t)pstride, nb);
-    kin = kin + block_sum_range(ps + 3 * (size_t)pstride, nb);
-  }
-  if (threadIdx.x == 0) {
-    if (measflag == 1) {
-      obs[OBS_A02] = r2 / vol;
-      obs[OBS_A04] = r4 / vol;
-      obs[OBS_A0KIN] = kin / (6.0 * vol);
-    }
-    double accept = acc / vol;
-    if (kind == 1) {
-      obs[OBS_ACC_HIGGS] = accept;
-      double d = obs[OBS_DELTA];
-      if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
-      obs[OBS_DELTA] = d;
-    } else if (measflag == 1) {
-      obs[OBS_ACC_HIGGS] = accept;
-    }
-  }
+  if (block_reduce_last<4>(red, p.red, blk) && p.final_ && threadIdx.x == 0)
+    higgs_observables(p, red);
 }
 
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
-                  uint64_t pass, int slot) {
+                  uint64_t pass, int slot, int final_, double* hist_row) {
   HiggsArgs a;
   a.g = L->g;
   a.U = L->U;
@@ -464,8 +561,13 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
   a.pass = pass;
   a.measflag = measflag;
-  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
-  a.pstride = L->pstride;
+  a.kind = kind;
+  a.final_ = final_;
+  a.red = slot_reduce(L, slot);
+  a.slot0_tot = L->slot_tot + (slot - 1) * LCEQ_NQ;
+  a.vol = (double)L->vol;
+  a.obs = L->obs;
+  a.hist_row = hist_row;
   a.nb = blocks_for(L->g.n);
   dim3 grid(a.nb), blk(TB);
   prof_begin(L, KC_HIGGS);
@@ -483,11 +585,6 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   }
 #undef LCEQ_HIGGS_LAUNCH
   prof_end(L, KC_HIGGS);
-}
-
-void launch_higgs_finalize(lceq_lattice* L, int kind, int measflag) {
-  hipLaunchKernelGGL(k_higgs_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials, L->pstride,
-                     blocks_for(L->g.n), (double)L->vol, kind, measflag, L->obs);
 }
 
 // =============================================================== renorm
@@ -513,9 +610,10 @@ struct GfArgs {
   double* Af;
   int eo, compute_da, renorm, guarded;
   double lambda, alpha;
-  const double* obs;  // OBS_GF_DONE read when guarded
-  double* part;
-  int pstride, nb;
+  double* obs;  // OBS_GF_DONE read when guarded; OBS_DA, OBS_ITER written
+  Reduce red;
+  double n, limit_or, max_iter;
+  int nb;
 };
 
 // One parity half of f/or_gauge_fixing1.f:54-328.  Thread x reads and
@@ -581,26 +679,20 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       st_link(p.Uf, g, mu, oe, sy[mu], ny);
     }
   }
-  if (p.compute_da) block_sum_store<1>(red, p.part, p.pstride, blk);
-}
-
-// da1 = sum/N (f/or_gauge_fixing1.f:156-157); for guarded (landau_gauge)
-// iterations also count the iteration and set the convergence flag
-// (f/landau_gauge.f:54-59) so later launches of the batch exit at once.
-__global__ __launch_bounds__(256) void k_gfix_finalize(const double* __restrict__ part, int nb,
-                                                       double n, int guarded, double limit_or,
-                                                       double max_iter, double* __restrict__ obs) {
-  if (guarded && obs[OBS_GF_DONE] != 0.0) return;
-  double da = block_sum_range(part, nb) / n;
-  if (threadIdx.x == 0) {
-    obs[OBS_DA] = da;
-    if (guarded) {
-      double it = obs[OBS_ITER] + 1.0;
-      obs[OBS_ITER] = it;
-      if (!(da > limit_or)) obs[OBS_GF_DONE] = 1.0;
-      else if (it >= max_iter) {
-        obs[OBS_GF_DONE] = 1.0;
-        obs[OBS_GF_CAPPED] = 1.0;
+  // da1 = sum/N (f/or_gauge_fixing1.f:156-157); guarded (landau_gauge)
+  // iterations also count the iteration and set the convergence flag
+  // (f/landau_gauge.f:54-59) so that the rest of the batch exits at once.
+  if (p.compute_da && block_reduce_last<1>(red, p.red, blk) && threadIdx.x == 0) {
+    double da = red[0] / p.n;
+    p.obs[OBS_DA] = da;
+    if (p.guarded) {
+      double it = p.obs[OBS_ITER] + 1.0;
+      p.obs[OBS_ITER] = it;
+      if (!(da > p.limit_or)) {
+        p.obs[OBS_GF_DONE] = 1.0;
+      } else if (it >= p.max_iter) {
+        p.obs[OBS_GF_DONE] = 1.0;
+        p.obs[OBS_GF_CAPPED] = 1.0;
       }
     }
   }
@@ -618,18 +710,14 @@ void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarde
   a.lambda = L->lambda;
   a.alpha = L->alpha;
   a.obs = L->obs;
-  a.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
-  a.pstride = L->pstride;
+  a.red = slot_reduce(L, slot);
+  a.n = (double)L->g.n;
+  a.limit_or = L->limit_or;
+  a.max_iter = (double)L->max_iter;
   a.nb = blocks_for(L->g.n);
   prof_begin(L, KC_GFIX);
   hipLaunchKernelGGL(k_gfix, dim3(a.nb), dim3(TB), 0, L->stream, a);
   prof_end(L, KC_GFIX);
-}
-
-void launch_gfix_finalize(lceq_lattice* L, int guarded) {
-  hipLaunchKernelGGL(k_gfix_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials,
-                     blocks_for(L->g.n), (double)L->g.n, guarded, L->limit_or,
-                     (double)L->max_iter, L->obs);
 }
 
 // ========================================================== correlators
@@ -927,15 +1015,6 @@ void launch_unitconf(lceq_lattice* L) {
 void launch_testconf(lceq_lattice* L) {
   hipLaunchKernelGGL(k_startconf, dim3(blocks_for(L->g.n), 8), dim3(TB), 0, L->stream, L->g, L->U,
                      L->A, 1);
-}
-
-// copy the observables into the per-sweep history row `index`
-__global__ void k_record(const double* __restrict__ obs, double* __restrict__ hist, int index) {
-  int i = threadIdx.x;
-  if (i < LCEQ_OBS_N) hist[(size_t)index * LCEQ_OBS_N + i] = obs[i];
-}
-void launch_record(lceq_lattice* L, int index) {
-  hipLaunchKernelGGL(k_record, dim3(1), dim3(64), 0, L->stream, L->obs, L->hist, index);
 }
 
 }  // namespace lceq
