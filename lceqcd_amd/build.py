@@ -20,6 +20,25 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("FC_LCEQ", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran():
+    """The iso_c_binding module and the Fortran host driver (needs amdflang).
+    Output: lceqcd_amd/fortran/lceq_main, linked against ../liblceq.so."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [FC, "-O2", "-J", td, os.path.join(FORTRAN, "lceq_c.f90"),
+               os.path.join(FORTRAN, "lceq_main.f90"), "-L", HERE, "-llceq",
+               "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(FORTRAN, "lceq_main")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed building the Fortran driver")
+    return os.path.join(FORTRAN, "lceq_main")
+
+
 def build(verbose=False):
     """Compile every HIP source into lceqcd_amd/liblceq.so (always rebuilds)."""
     tmp = OUT + ".tmp"
@@ -31,6 +50,8 @@ def build(verbose=False):
     if verbose:
         sys.stderr.write(r.stderr)
     os.replace(tmp, OUT)
+    if os.path.exists(FC):
+        build_fortran()
     return OUT
 
 

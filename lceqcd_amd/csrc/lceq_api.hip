@@ -91,9 +91,7 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   alloc(&L->Vst, 24 * np);
   alloc(&L->F, 3 * np);
   alloc(&L->partials, (size_t)LCEQ_NSLOT * LCEQ_NQ * L->pstride);
-  alloc(&L->slot_tot, (size_t)LCEQ_NSLOT * LCEQ_NQ);
-  if (rc == LCEQ_OK && hipMalloc((void**)&L->counters, LCEQ_NSLOT * sizeof(unsigned)) != hipSuccess)
-    rc = fail(LCEQ_ERR_HIP, "hipMalloc(counters) failed");
+
   alloc(&L->obs, LCEQ_OBS_N);
   alloc(&L->slices, 14 * (size_t)lz);
   alloc(&L->corr, 5 * (size_t)lz + 8);
@@ -110,8 +108,7 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
     (void)hipMemsetAsync(L->partials, 0, (size_t)LCEQ_NSLOT * LCEQ_NQ * L->pstride * sizeof(double),
                    L->stream);
     (void)hipMemsetAsync(L->obs, 0, LCEQ_OBS_N * sizeof(double), L->stream);
-    (void)hipMemsetAsync(L->slot_tot, 0, (size_t)LCEQ_NSLOT * LCEQ_NQ * sizeof(double), L->stream);
-    (void)hipMemsetAsync(L->counters, 0, LCEQ_NSLOT * sizeof(unsigned), L->stream);
+
     if (hipStreamSynchronize(L->stream) != hipSuccess) rc = fail(LCEQ_ERR_HIP, "init sync failed");
   }
   if (rc != LCEQ_OK) {
@@ -125,9 +122,8 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
 extern "C" int lceq_destroy(lceq_lattice* L) {
   if (!L) return LCEQ_OK;
   if (L->stream) (void)hipStreamSynchronize(L->stream);
-  double* bufs[] = {L->U, L->A, L->Uf, L->Af, L->Vst, L->F, L->partials, L->slot_tot, L->obs,
+  double* bufs[] = {L->U, L->A, L->Uf, L->Af, L->Vst, L->F, L->partials, L->obs,
                     L->slices, L->corr, L->hist, L->scratch, L->fed};
-  if (L->counters) (void)hipFree(L->counters);
   for (double* b : bufs)
     if (b) (void)hipFree(b);
   if (L->prof_ev) {
@@ -388,10 +384,9 @@ extern "C" int lceq_heat_kpi(lceq_lattice* L, int n, const double* b, double* v,
 }
 
 // one full link sweep (eo outer, mu inner as in f/agauge.f:53-57); with
-// reduce the six launches accumulate the plaquette and acceptance and the
-// last one writes the observables (and the history row, if given)
-static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce,
-                      double* hist_row) {
+// reduce the six launches leave their plaquette and acceptance partial sums
+// in slots 0-5 for launch_finalize
+static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce) {
   int64_t per = (kp ? 7 : 1) * (int64_t)L->g.n;
   const double* fed0;
   int rc = take_uniforms(L, 6 * per, &fed0);
@@ -399,17 +394,17 @@ static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write
   for (int eo = 0; eo < 2; ++eo)
     for (int mu = 0; mu < 3; ++mu) {
       int slot = eo * 3 + mu;
-      int last = slot == 5;
       launch_link(L, kp, eo, mu, measflag, renorm_on_write, fed0 ? fed0 + slot * per : nullptr,
-                  next_pass(L), slot, reduce, reduce && last, last ? hist_row : nullptr);
+                  next_pass(L), reduce);
     }
   return check_launch();
 }
 
 static int link_call(lceq_lattice* L, int kp, int measflag, double* accept) {
   NEEDL(L);
-  int rc = link_sweep(L, kp, measflag, 0, 1, nullptr);
+  int rc = link_sweep(L, kp, measflag, 0, 1);
   if (rc) return rc;
+  launch_finalize(L, kp ? 1 : 2, measflag, 0, 0, nullptr);
   if (accept) {
     double o[LCEQ_OBS_N];
     if ((rc = read_obs(L, o))) return rc;
@@ -449,15 +444,15 @@ extern "C" int lceq_gauss(lceq_lattice* L, double* x, const double* r, int nh) {
   return LCEQ_OK;
 }
 
-// both parities of one Higgs routine (kind 1/2/3 = ahiggs1/2/3)
-static int higgs_sweep(lceq_lattice* L, int kind, int measflag, double* hist_row) {
+// both parities of one Higgs routine (kind 1/2/3 = ahiggs1/2/3); partial
+// sums land in slots 6-7 for launch_finalize
+static int higgs_sweep(lceq_lattice* L, int kind, int measflag) {
   int64_t per = (kind == 1 ? 4 : kind == 2 ? 7 : 1) * (int64_t)L->g.n;
   const double* fed0;
   int rc = take_uniforms(L, 2 * per, &fed0);
   if (rc) return rc;
   for (int eo = 0; eo < 2; ++eo)
-    launch_higgs(L, kind, eo, measflag, fed0 ? fed0 + eo * per : nullptr, next_pass(L), eo, eo == 1,
-                 eo == 1 ? hist_row : nullptr);
+    launch_higgs(L, kind, eo, measflag, fed0 ? fed0 + eo * per : nullptr, next_pass(L));
   return check_launch();
 }
 
@@ -466,7 +461,8 @@ extern "C" int lceq_ahiggs1(lceq_lattice* L, double* delta, int measflag, double
   if (!delta) return fail(LCEQ_ERR_ARG, "delta is required");
   int rc = write_obs(L, OBS_DELTA, *delta);
   if (rc) return rc;
-  if ((rc = higgs_sweep(L, 1, measflag, nullptr))) return rc;
+  if ((rc = higgs_sweep(L, 1, measflag))) return rc;
+  launch_finalize(L, 0, 0, 1, measflag, nullptr);
   double o[LCEQ_OBS_N];
   if ((rc = read_obs(L, o))) return rc;
   *delta = o[OBS_DELTA];
@@ -476,8 +472,9 @@ extern "C" int lceq_ahiggs1(lceq_lattice* L, double* delta, int measflag, double
 
 static int higgs_call(lceq_lattice* L, int kind, int measflag, double* accept) {
   NEEDL(L);
-  int rc = higgs_sweep(L, kind, measflag, nullptr);
+  int rc = higgs_sweep(L, kind, measflag);
   if (rc) return rc;
+  launch_finalize(L, 0, 0, kind, measflag, nullptr);
   if (accept && measflag == 1) {
     double o[LCEQ_OBS_N];
     if ((rc = read_obs(L, o))) return rc;
@@ -505,8 +502,9 @@ extern "C" int lceq_u_f_renorm(lceq_lattice* L) {
 
 extern "C" int lceq_or_gauge_fixing(lceq_lattice* L, double* da1) {
   NEEDL(L);
-  launch_gfix(L, 0, 0, 0, 0, 0);
-  launch_gfix(L, 1, 1, 0, 0, 0);
+  launch_gfix(L, 0, 0, 0, 0);
+  launch_gfix(L, 1, 1, 0, 0);
+  launch_gfix_finalize(L, 0);
   int rc = check_launch();
   if (rc) return rc;
   if (da1) {
@@ -533,8 +531,9 @@ extern "C" int lceq_landau_gauge(lceq_lattice* L, double* da1, int* iter_or1) {
   double o[LCEQ_OBS_N];
   for (;;) {
     for (int k = 0; k < batch; ++k) {
-      launch_gfix(L, 0, 0, 0, 1, 0);
-      launch_gfix(L, 1, 1, 1, 1, 0);
+      launch_gfix(L, 0, 0, 0, 1);
+      launch_gfix(L, 1, 1, 1, 1);
+      launch_gfix_finalize(L, 1);
     }
     int rc = check_launch();
     if (rc) return rc;
@@ -663,18 +662,22 @@ extern "C" int lceq_sweep(lceq_lattice* L, int schedule, int nsweep, int n_or, d
     double* row = measflag ? L->hist + (size_t)(L->hist_rows++) * LCEQ_OBS_N : nullptr;
     if (schedule == LCEQ_SWEEP_KP_OR_METRO) {
       // only the last link sweep carries the measurement; the ahiggs1 step
-      // adapts every sweep, so the Higgs launches always reduce
+      // adapts every sweep, so the Higgs launches always reduce.  One
+      // finalize launch closes the sweep.
       int meas_kp = measflag && n_or == 0;
-      if ((rc = link_sweep(L, 1, meas_kp, 1, meas_kp, nullptr))) return rc;
+      if ((rc = link_sweep(L, 1, meas_kp, 1, meas_kp))) return rc;
       for (int k = 0; k < n_or; ++k) {
         int m = measflag && k == n_or - 1;
-        if ((rc = link_sweep(L, 0, m, 1, m, nullptr))) return rc;
+        if ((rc = link_sweep(L, 0, m, 1, m))) return rc;
       }
-      if ((rc = higgs_sweep(L, 1, measflag, row))) return rc;
+      if ((rc = higgs_sweep(L, 1, measflag))) return rc;
+      launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 1, measflag, row);
     } else {  // f/main.f:38-41
-      if ((rc = higgs_sweep(L, 2, measflag, nullptr))) return rc;
-      if ((rc = higgs_sweep(L, 3, measflag, nullptr))) return rc;
-      if ((rc = link_sweep(L, 1, measflag, 0, 1, row))) return rc;
+      if ((rc = higgs_sweep(L, 2, measflag))) return rc;
+      if (measflag) launch_finalize(L, 0, 0, 2, 1, nullptr);
+      if ((rc = higgs_sweep(L, 3, measflag))) return rc;
+      if ((rc = link_sweep(L, 1, measflag, 0, measflag))) return rc;
+      if (measflag) launch_finalize(L, 1, 1, 3, 1, row);
       launch_renorm(L, L->U);
     }
   }

@@ -225,44 +225,6 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restr
   }
 }
 
-// One launch's grid-wide reduction slot: per-block partials, an arrival
-// counter (zero between launches) and the slot totals.
-struct Reduce {
-  double* part;       // [NQ][pstride]
-  unsigned* counter;
-  double* tot;        // [NQ]
-  int pstride, nb;
-};
-
-// Store this block's partials; the block that arrives last sums all nb
-// partials of each quantity in a fixed order (so the result does not depend
-// on arrival order), stores the totals, re-arms the counter and returns true
-// with the totals in v.  Replaces a separate reduction launch.
-template <int NQ>
-__device__ __forceinline__ bool block_reduce_last(double (&v)[NQ], const Reduce& r, int blk) {
-  __shared__ int is_last;
-  block_sum_store<NQ>(v, r.part, r.pstride, blk);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = (atomicAdd(r.counter, 1u) == (unsigned)(r.nb - 1));
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) v[q] = 0.0;
-  for (int i = threadIdx.x; i < r.nb; i += 256) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) v[q] += r.part[(size_t)q * r.pstride + i];
-  }
-  block_sum<NQ>(v);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) r.tot[q] = v[q];
-    *r.counter = 0u;
-  }
-  return true;
-}
-
 // Blocks are dealt round-robin to the 8 XCDs; hand each XCD a contiguous
 // range of logical blocks so that neighbour gathers (z-neighbours are
 // lx*ly/2 sites away) stay in that XCD's L2.

@@ -39,8 +39,6 @@ struct lceq_lattice {
   double *U, *A, *Uf, *Af, *Vst, *F;
   double *partials;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
   int pstride;       // >= number of 256-site blocks
-  unsigned *counters;  // [LCEQ_NSLOT] arrival counters of the last-block reductions
-  double *slot_tot;  // [LCEQ_NSLOT][LCEQ_NQ] per-launch totals
   double *obs;       // [LCEQ_OBS_N]
   double *hist;      // [hist_cap][LCEQ_OBS_N] per-sweep records
   int hist_cap;
@@ -70,18 +68,22 @@ enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC
 int blocks_for(int n);
 
 // launchers (lceq_kernels.hip); all enqueue on L->stream, no host sync.
-// reduce: accumulate the launch's partial sums into slot `slot`; final: this
-// launch closes its routine and writes the observables (and, with hist_row,
-// one history row).
+// Link launches use reduction slots 0-5 ((eo, mu) order), Higgs launches
+// slots 6-7 (eo); with reduce = 0 a launch writes no partial sums.
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
-                 const double* fed, uint64_t pass, int slot, int reduce, int final_,
-                 double* hist_row);
+                 const double* fed, uint64_t pass, int reduce);
+// One-block reduction of the partial sums into the observables: link_kind
+// 0 none, 1 KP (agauge), 2 overrelaxation; higgs_kind 0 none, 1-3 ahiggs1-3;
+// with hist_row the observables are also copied into that history row.
+void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
+                     int higgs_meas, double* hist_row);
 void launch_staples(lceq_lattice* L, int eo, int mu);
 void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
-                  uint64_t pass, int slot, int final_, double* hist_row);
+                  uint64_t pass);
 void launch_renorm(lceq_lattice* L, double* U);
-void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int slot);
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded);
+void launch_gfix_finalize(lceq_lattice* L, int guarded);
 void launch_slices(lceq_lattice* L);
 void launch_corr(lceq_lattice* L);
 void launch_heat_kpi(lceq_lattice* L, int n, const double* b, const double* a0_old, double* v,

@@ -48,40 +48,10 @@ struct LinkArgs {
   Philox key;
   uint64_t pass;
   int measflag, renorm;
-  int reduce, final_, slot;
-  Reduce red;
-  const double* slot_tot;  // [LCEQ_NSLOT][LCEQ_NQ] totals of this routine's earlier launches
-  double vol;
-  double* obs;
-  double* hist_row;        // copy the observables here when final_ (may be null)
-  int nb;
+  int reduce;
+  double* part;  // partial sums of this launch's slot: [LCEQ_NQ][pstride]
+  int pstride, nb;
 };
-
-// Observables of a complete link sweep from the six (eo, mu) slot totals in
-// reference order (f/agauge.f:195-200); slot 5's totals are in last[].
-__device__ void link_observables(const LinkArgs& p, const double (&last)[3]) {
-  double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
-  for (int slot = 0; slot < 6; ++slot) {
-    const double* t = slot == 5 ? last : p.slot_tot + slot * LCEQ_NQ;
-    plaq = plaq + t[0];
-    nacch += t[1];
-    acc1 += t[2];
-  }
-  double* obs = p.obs;
-  obs[OBS_NACCH] = nacch;
-  obs[OBS_NACC1] = acc1;
-  if (p.measflag == 1) obs[OBS_PLAQ0] = plaq / (12.0 * p.vol);
-  double accept;
-  if (p.final_ == 1) {  // Kennedy-Pendleton: (nacch/3V)(acc1/12V) with measflag, else nacch
-    accept = nacch;
-    if (p.measflag == 1) accept = (accept / (3.0 * p.vol)) * (acc1 / (12.0 * p.vol));
-  } else {              // overrelaxation: Metropolis acceptance
-    accept = acc1 / (3.0 * p.vol);
-  }
-  obs[OBS_ACC_LINK] = accept;
-  if (p.hist_row)
-    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
-}
 
 // The 12 links of the staple V(x) of f/staples.f:65-216 for x in parity eo,
 // direction mu, two nu per site: t = 0, 1 for nu = the two directions != mu
@@ -245,23 +215,11 @@ __global__ __launch_bounds__(256) void k_link(LinkArgs p) {
     }
     if (p.measflag == 1) red[0] = m_nn(u, V).a.re;  // f/agauge.f:179-185
   }
-  if (p.reduce && block_reduce_last<3>(red, p.red, blk) && p.final_ && threadIdx.x == 0)
-    link_observables(p, red);
-}
-
-static Reduce slot_reduce(lceq_lattice* L, int slot) {
-  Reduce r;
-  r.part = L->partials + (size_t)slot * LCEQ_NQ * L->pstride;
-  r.counter = L->counters + slot;
-  r.tot = L->slot_tot + slot * LCEQ_NQ;
-  r.pstride = L->pstride;
-  r.nb = blocks_for(L->g.n);
-  return r;
+  if (p.reduce) block_sum_store<3>(red, p.part, p.pstride, blk);
 }
 
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
-                 const double* fed, uint64_t pass, int slot, int reduce, int final_,
-                 double* hist_row) {
+                 const double* fed, uint64_t pass, int reduce) {
   LinkArgs a;
   a.g = L->g;
   a.U = L->U;
@@ -276,13 +234,8 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.measflag = measflag;
   a.renorm = renorm_on_write;
   a.reduce = reduce;
-  a.final_ = final_ ? (kp ? 1 : 2) : 0;
-  a.slot = slot;
-  a.red = slot_reduce(L, slot);
-  a.slot_tot = L->slot_tot;
-  a.vol = (double)L->vol;
-  a.obs = L->obs;
-  a.hist_row = hist_row;
+  a.part = L->partials + (size_t)(eo * 3 + mu) * LCEQ_NQ * L->pstride;
+  a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
@@ -298,6 +251,103 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
       hipLaunchKernelGGL((k_link<false, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
   }
   prof_end(L, kc);
+}
+
+// ============================================================ finalize
+
+struct FinalizeArgs {
+  const double* part;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
+  int pstride, nb;
+  int link_kind, link_meas, higgs_kind, higgs_meas;
+  double vol;
+  double* obs;
+  double* hist_row;
+};
+
+// One block sums every partial the closing routines wrote -- six link slots
+// x 3 quantities and two Higgs slots x 4 quantities, all loads in flight at
+// once -- per slot in a fixed order, then combines the slots in the
+// reference's loop order: f/agauge.f:195-200 (links), f/ahiggs1.f:107-123 and
+// f/ahiggs2.f:138-143 (Higgs).
+__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs p) {
+  constexpr int NV = 6 * 3 + 2 * 4;
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  for (int i = threadIdx.x; i < p.nb; i += TB) {
+    if (p.link_kind) {
+#pragma unroll
+      for (int slot = 0; slot < 6; ++slot)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          v[slot * 3 + q] += p.part[((size_t)slot * LCEQ_NQ + q) * p.pstride + i];
+    }
+    if (p.higgs_kind) {
+#pragma unroll
+      for (int slot = 0; slot < 2; ++slot)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          v[18 + slot * 4 + q] += p.part[((size_t)(6 + slot) * LCEQ_NQ + q) * p.pstride + i];
+    }
+  }
+  block_sum<NV>(v);
+  if (threadIdx.x != 0) return;
+  double* obs = p.obs;
+  if (p.link_kind) {
+    double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
+    for (int slot = 0; slot < 6; ++slot) {
+      plaq = plaq + v[slot * 3];
+      nacch += v[slot * 3 + 1];
+      acc1 += v[slot * 3 + 2];
+    }
+    obs[OBS_NACCH] = nacch;
+    obs[OBS_NACC1] = acc1;
+    if (p.link_meas == 1) obs[OBS_PLAQ0] = plaq / (12.0 * p.vol);
+    double accept;
+    if (p.link_kind == 1) {  // (nacch/3V)(acc1/12V) with measflag, else nacch
+      accept = nacch;
+      if (p.link_meas == 1) accept = (accept / (3.0 * p.vol)) * (acc1 / (12.0 * p.vol));
+    } else {
+      accept = acc1 / (3.0 * p.vol);
+    }
+    obs[OBS_ACC_LINK] = accept;
+  }
+  if (p.higgs_kind) {
+    double acc = v[18] + v[22];
+    double r2 = v[19] + v[23], r4 = v[20] + v[24], kin = v[21] + v[25];
+    if (p.higgs_meas == 1) {
+      obs[OBS_A02] = r2 / p.vol;
+      obs[OBS_A04] = r4 / p.vol;
+      obs[OBS_A0KIN] = kin / (6.0 * p.vol);
+    }
+    double accept = acc / p.vol;
+    if (p.higgs_kind == 1) {
+      obs[OBS_ACC_HIGGS] = accept;
+      double d = obs[OBS_DELTA];
+      if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+      obs[OBS_DELTA] = d;
+    } else if (p.higgs_meas == 1) {
+      obs[OBS_ACC_HIGGS] = accept;
+    }
+  }
+  if (p.hist_row)
+    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
+}
+
+void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
+                     int higgs_meas, double* hist_row) {
+  FinalizeArgs a;
+  a.part = L->partials;
+  a.pstride = L->pstride;
+  a.nb = blocks_for(L->g.n);
+  a.link_kind = link_kind;
+  a.link_meas = link_meas;
+  a.higgs_kind = higgs_kind;
+  a.higgs_meas = higgs_meas;
+  a.vol = (double)L->vol;
+  a.obs = L->obs;
+  a.hist_row = hist_row;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TB), 0, L->stream, a);
 }
 
 // staple only, for the staples() entry point (f/staples.f:9)
@@ -388,38 +438,10 @@ struct HiggsArgs {
   const double* fed;
   Philox key;
   uint64_t pass;
-  int measflag, kind, final_;
-  Reduce red;
-  const double* slot0_tot;  // totals of the eo = 0 launch
-  double vol;
-  double* obs;
-  double* hist_row;
-  int nb;
+  int measflag;
+  double* part;
+  int pstride, nb;
 };
-
-// f/ahiggs1.f:107-123, f/ahiggs2.f:138-143 from the two parity totals
-__device__ void higgs_observables(const HiggsArgs& p, const double (&last)[4]) {
-  const double* t0 = p.slot0_tot;
-  double acc = t0[0] + last[0];
-  double r2 = t0[1] + last[1], r4 = t0[2] + last[2], kin = t0[3] + last[3];
-  double* obs = p.obs;
-  if (p.measflag == 1) {
-    obs[OBS_A02] = r2 / p.vol;
-    obs[OBS_A04] = r4 / p.vol;
-    obs[OBS_A0KIN] = kin / (6.0 * p.vol);
-  }
-  double accept = acc / p.vol;
-  if (p.kind == 1) {
-    obs[OBS_ACC_HIGGS] = accept;
-    double d = obs[OBS_DELTA];
-    if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
-    obs[OBS_DELTA] = d;
-  } else if (p.measflag == 1) {
-    obs[OBS_ACC_HIGGS] = accept;
-  }
-  if (p.hist_row)
-    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
-}
 
 // KIND 1: f/ahiggs1.f Metropolis; 2: f/ahiggs2.f heatbath; 3: f/ahiggs3.f
 // overrelaxation.  Partials: 0 accepts, 1 rho, 2 rho^2, 3 a.F (post update).
@@ -541,12 +563,11 @@ __global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
       red[3] = (a.x[0] * F.x[0] + a.x[1] * F.x[1]) + a.x[2] * F.x[2];
     }
   }
-  if (block_reduce_last<4>(red, p.red, blk) && p.final_ && threadIdx.x == 0)
-    higgs_observables(p, red);
+  block_sum_store<4>(red, p.part, p.pstride, blk);
 }
 
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
-                  uint64_t pass, int slot, int final_, double* hist_row) {
+                  uint64_t pass) {
   HiggsArgs a;
   a.g = L->g;
   a.U = L->U;
@@ -561,13 +582,8 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
   a.pass = pass;
   a.measflag = measflag;
-  a.kind = kind;
-  a.final_ = final_;
-  a.red = slot_reduce(L, slot);
-  a.slot0_tot = L->slot_tot + (slot - 1) * LCEQ_NQ;
-  a.vol = (double)L->vol;
-  a.obs = L->obs;
-  a.hist_row = hist_row;
+  a.part = L->partials + (size_t)(6 + eo) * LCEQ_NQ * L->pstride;
+  a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
   dim3 grid(a.nb), blk(TB);
   prof_begin(L, KC_HIGGS);
@@ -610,10 +626,9 @@ struct GfArgs {
   double* Af;
   int eo, compute_da, renorm, guarded;
   double lambda, alpha;
-  double* obs;  // OBS_GF_DONE read when guarded; OBS_DA, OBS_ITER written
-  Reduce red;
-  double n, limit_or, max_iter;
-  int nb;
+  const double* obs;  // OBS_GF_DONE read when guarded
+  double* part;
+  int pstride, nb;
 };
 
 // One parity half of f/or_gauge_fixing1.f:54-328.  Thread x reads and
@@ -679,26 +694,36 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       st_link(p.Uf, g, mu, oe, sy[mu], ny);
     }
   }
-  // da1 = sum/N (f/or_gauge_fixing1.f:156-157); guarded (landau_gauge)
-  // iterations also count the iteration and set the convergence flag
-  // (f/landau_gauge.f:54-59) so that the rest of the batch exits at once.
-  if (p.compute_da && block_reduce_last<1>(red, p.red, blk) && threadIdx.x == 0) {
-    double da = red[0] / p.n;
-    p.obs[OBS_DA] = da;
-    if (p.guarded) {
-      double it = p.obs[OBS_ITER] + 1.0;
-      p.obs[OBS_ITER] = it;
-      if (!(da > p.limit_or)) {
-        p.obs[OBS_GF_DONE] = 1.0;
-      } else if (it >= p.max_iter) {
-        p.obs[OBS_GF_DONE] = 1.0;
-        p.obs[OBS_GF_CAPPED] = 1.0;
+  if (p.compute_da) block_sum_store<1>(red, p.part, p.pstride, blk);
+}
+
+// da1 = sum/N (f/or_gauge_fixing1.f:156-157); for guarded (landau_gauge)
+// iterations also count the iteration and set the convergence flag
+// (f/landau_gauge.f:54-59) so that the rest of the batch exits at once.
+__global__ __launch_bounds__(256) void k_gfix_finalize(const double* __restrict__ part, int nb,
+                                                       double n, int guarded, double limit_or,
+                                                       double max_iter, double* __restrict__ obs) {
+  if (guarded && obs[OBS_GF_DONE] != 0.0) return;
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < nb; i += TB) v[0] += part[i];
+  block_sum<1>(v);
+  if (threadIdx.x == 0) {
+    double da = v[0] / n;
+    obs[OBS_DA] = da;
+    if (guarded) {
+      double it = obs[OBS_ITER] + 1.0;
+      obs[OBS_ITER] = it;
+      if (!(da > limit_or)) {
+        obs[OBS_GF_DONE] = 1.0;
+      } else if (it >= max_iter) {
+        obs[OBS_GF_DONE] = 1.0;
+        obs[OBS_GF_CAPPED] = 1.0;
       }
     }
   }
 }
 
-void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int slot) {
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded) {
   GfArgs a;
   a.g = L->g;
   a.Uf = L->Uf;
@@ -710,14 +735,18 @@ void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarde
   a.lambda = L->lambda;
   a.alpha = L->alpha;
   a.obs = L->obs;
-  a.red = slot_reduce(L, slot);
-  a.n = (double)L->g.n;
-  a.limit_or = L->limit_or;
-  a.max_iter = (double)L->max_iter;
+  a.part = L->partials;
+  a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
   prof_begin(L, KC_GFIX);
   hipLaunchKernelGGL(k_gfix, dim3(a.nb), dim3(TB), 0, L->stream, a);
   prof_end(L, KC_GFIX);
+}
+
+void launch_gfix_finalize(lceq_lattice* L, int guarded) {
+  hipLaunchKernelGGL(k_gfix_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials,
+                     blocks_for(L->g.n), (double)L->g.n, guarded, L->limit_or,
+                     (double)L->max_iter, L->obs);
 }
 
 // ========================================================== correlators

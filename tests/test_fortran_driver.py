@@ -1,0 +1,84 @@
+"""The Fortran host driver (lceqcd_amd/fortran/lceq_main.f90) over the C ABI:
+builds with amdflang here; on the GPU it runs the f/main.f schedule on an 8^3
+lattice, writes the reference's output files, and its action records equal
+the CPU restatement's run of the same schedule and stream."""
+import glob
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FDIR = os.path.join(ROOT, "lceqcd_amd", "fortran")
+FC = "/opt/rocm/bin/amdflang"
+
+# the reference's 20-line inputfile layout (runs/inputfile), at 8^3
+INPUTFILE = """08 08 08      !nx/ny/nz global lattice size
+01 01 01      !dimproc
+001           !nproc
+{init}            ! init
+0             !rand
+0             !save
+000000        !confstart
+ 9.000        !beta
+0.1000        !x
+0.5000        !y
+1.7000        !lambda overrelaxation parameter
+1.000000      !gauge transformation parameter
+0.100E-09     !limit_or
+0.100E-04     !limit_it
+001           !inorm
+0             !nsweep_flag
+240           !ntime
+{nsweep}      !nsweeps
+00001         !imeas
+000           !imode
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not installed")
+def test_driver_compiles(tmp_path):
+    r = subprocess.run([FC, "-O2", "-J", str(tmp_path), os.path.join(FDIR, "lceq_c.f90"),
+                        os.path.join(FDIR, "lceq_main.f90"), "-c"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_runs_reference_schedule(tmp_path, oracle_lib):
+    exe = os.path.join(FDIR, "lceq_main")
+    assert os.path.exists(exe), "lceq_main not built (python -m lceqcd_amd.build)"
+    (tmp_path / "inputfile").write_text(INPUTFILE.format(init=0, nsweep="00002"))
+    print("running lceq_main 8^3 nsweep=2 ntherm=0", flush=True)
+    r = subprocess.run([exe, "inputfile", "4242", "0"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    act = glob.glob(str(tmp_path / "action_s08_be09000_x01000_y05000_u000000"))
+    assert act
+    lines = open(act[0]).read().splitlines()
+    assert lines[0].startswith("# 08 08 08") and len(lines) == 5
+    rec = np.array([[float(x) for x in l.split()[1:]] for l in lines[3:]])
+    corr = open(glob.glob(str(tmp_path / "correl_s08_*"))[0]).read().splitlines()
+    assert len(corr) == 3 + 2 * (1 + 8)
+    job = open(glob.glob(str(tmp_path / "joblog_s08_*"))[0]).read()
+    assert "Gauge  acceptance" in job and "number of iteration for or. gauge fixing" in job
+    # the same schedule on the CPU restatement (couplings of f/paraminit.f:150-153)
+    from lceqcd_amd import couplings_from_xy
+    O = oracle_lib.OracleLattice(8, 8, 8)
+    O.set_couplings(9.0, *couplings_from_xy(9.0, 0.1, 0.5))
+    O.hotstart(4242, 0.5)
+    O.philox(4242, 0)
+    exp = []
+    for it in range(2):
+        O.ahiggs2(1)
+        O.ahiggs3(1)
+        O.agauge(1)
+        O.renorm()
+        ob = O.observables()
+        exp.append([ob["plaq0"], ob["a0kin"], ob["a02"], ob["a04"]])
+        O.set_gauge_fixing(1.7, 1.0, 1e-10)
+        O.landau_gauge()
+        O.correlator()
+    np.testing.assert_allclose(rec, np.array(exp), rtol=1e-12)
