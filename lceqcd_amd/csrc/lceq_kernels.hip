@@ -264,34 +264,37 @@ struct FinalizeArgs {
   double* hist_row;
 };
 
-// One block sums every partial the closing routines wrote -- six link slots
-// x 3 quantities and two Higgs slots x 4 quantities, all loads in flight at
-// once -- per slot in a fixed order, then combines the slots in the
-// reference's loop order: f/agauge.f:195-200 (links), f/ahiggs1.f:107-123 and
-// f/ahiggs2.f:138-143 (Higgs).
-__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs p) {
+// One 1024-thread block sums every partial the closing routines wrote: six
+// link slots x 3 quantities and two Higgs slots x 4 quantities.  Wave w owns
+// quantities w and w+16; each lane sums its strided share of the block
+// partials (all loads in flight at once), a xor-butterfly finishes the wave
+// sum (fixed order, so the result is reproducible), and thread 0 combines the
+// slots in the reference's loop order: f/agauge.f:195-200 (links),
+// f/ahiggs1.f:107-123 and f/ahiggs2.f:138-143 (Higgs).
+constexpr int FIN_TB = 1024;
+__global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
   constexpr int NV = 6 * 3 + 2 * 4;
-  double v[NV];
+  __shared__ double tot[32];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = 0.0;
-  for (int i = threadIdx.x; i < p.nb; i += TB) {
-    if (p.link_kind) {
+  for (int h = 0; h < 2; ++h) {
+    int q = w + 16 * h;  // 0..17 link (slot*3 + quantity), 18..25 Higgs
+    if (q >= NV) continue;
+    bool on = q < 18 ? p.link_kind != 0 : p.higgs_kind != 0;
+    double t = 0.0;
+    if (on) {
+      size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
+                          : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
+      const double* x = p.part + row * p.pstride;
+      for (int i = lane; i < p.nb; i += 64) t += x[i];
 #pragma unroll
-      for (int slot = 0; slot < 6; ++slot)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-          v[slot * 3 + q] += p.part[((size_t)slot * LCEQ_NQ + q) * p.pstride + i];
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     }
-    if (p.higgs_kind) {
-#pragma unroll
-      for (int slot = 0; slot < 2; ++slot)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          v[18 + slot * 4 + q] += p.part[((size_t)(6 + slot) * LCEQ_NQ + q) * p.pstride + i];
-    }
+    if (lane == 0) tot[q] = t;
   }
-  block_sum<NV>(v);
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  const double* v = tot;
   double* obs = p.obs;
   if (p.link_kind) {
     double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(256) void k_finalize(FinalizeArgs p) {
 }
 
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
-                     int higgs_meas, double* hist_row) {
+                     int higgs_meas, double* hist_row, hipStream_t stream) {
   FinalizeArgs a;
   a.part = L->partials;
   a.pstride = L->pstride;
@@ -347,7 +350,7 @@ void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_ki
   a.vol = (double)L->vol;
   a.obs = L->obs;
   a.hist_row = hist_row;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(TB), 0, L->stream, a);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_TB), 0, stream ? stream : L->stream, a);
 }
 
 // staple only, for the staples() entry point (f/staples.f:9)
