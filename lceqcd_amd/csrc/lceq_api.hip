@@ -84,11 +84,7 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   if (hipGetDevice(&L->device) != hipSuccess) rc = fail(LCEQ_ERR_HIP, "no HIP device");
   if (rc == LCEQ_OK && hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(LCEQ_ERR_HIP, "hipStreamCreate failed");
-  if (rc == LCEQ_OK && hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking) != hipSuccess)
-    rc = fail(LCEQ_ERR_HIP, "hipStreamCreate failed");
-  if (rc == LCEQ_OK && (hipEventCreateWithFlags(&L->ev_higgs, hipEventDisableTiming) != hipSuccess ||
-                        hipEventCreateWithFlags(&L->ev_fin, hipEventDisableTiming) != hipSuccess))
-    rc = fail(LCEQ_ERR_HIP, "hipEventCreate failed");
+
   alloc(&L->U, 24 * np);
   alloc(&L->A, 6 * np);
   alloc(&L->Uf, 24 * np);
@@ -135,10 +131,6 @@ extern "C" int lceq_destroy(lceq_lattice* L) {
     for (int i = 0; i < L->prof_cap; ++i) (void)hipEventDestroy(L->prof_ev[i]);
     delete[] L->prof_ev;
   }
-  if (L->side) (void)hipStreamSynchronize(L->side);
-  if (L->ev_higgs) (void)hipEventDestroy(L->ev_higgs);
-  if (L->ev_fin) (void)hipEventDestroy(L->ev_fin);
-  if (L->side) (void)hipStreamDestroy(L->side);
   if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
   return LCEQ_OK;
@@ -670,31 +662,17 @@ extern "C" int lceq_sweep(lceq_lattice* L, int schedule, int nsweep, int n_or, d
   for (int it = 0; it < nsweep; ++it) {
     double* row = measflag ? L->hist + (size_t)(L->hist_rows++) * LCEQ_OBS_N : nullptr;
     if (schedule == LCEQ_SWEEP_KP_OR_METRO) {
-      // Only the last link sweep carries the measurement; the ahiggs1 step
-      // adapts every sweep, so the Higgs launches always reduce.  The
-      // finalize of a sweep runs on the side stream, overlapping the next
-      // sweep's first link launches; the next measured link sweep (which
-      // rewrites the partials) and everything after it wait for it.
+      // only the last link sweep carries the measurement; the ahiggs1 step
+      // adapts every sweep, so the Higgs launches always reduce.  One
+      // finalize launch closes the sweep.
       int meas_kp = measflag && n_or == 0;
-      for (int k = -1; k < n_or; ++k) {
-        int kp = k < 0, m = kp ? meas_kp : (measflag && k == n_or - 1);
-        int writes_partials = m || k == n_or - 1;
-        if (writes_partials && L->fin_pending) {
-          HIPCHK(hipStreamWaitEvent(L->stream, L->ev_fin, 0));
-          L->fin_pending = 0;
-        }
-        if ((rc = link_sweep(L, kp, m, 1, m))) return rc;
-      }
-      if (L->fin_pending) {
-        HIPCHK(hipStreamWaitEvent(L->stream, L->ev_fin, 0));
-        L->fin_pending = 0;
+      if ((rc = link_sweep(L, 1, meas_kp, 1, meas_kp))) return rc;
+      for (int k = 0; k < n_or; ++k) {
+        int m = measflag && k == n_or - 1;
+        if ((rc = link_sweep(L, 0, m, 1, m))) return rc;
       }
       if ((rc = higgs_sweep(L, 1, measflag))) return rc;
-      HIPCHK(hipEventRecord(L->ev_higgs, L->stream));
-      HIPCHK(hipStreamWaitEvent(L->side, L->ev_higgs, 0));
-      launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 1, measflag, row, L->side);
-      HIPCHK(hipEventRecord(L->ev_fin, L->side));
-      L->fin_pending = 1;
+      launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 1, measflag, row);
     } else {  // f/main.f:38-41
       if ((rc = higgs_sweep(L, 2, measflag))) return rc;
       if (measflag) launch_finalize(L, 0, 0, 2, 1, nullptr);
@@ -705,10 +683,6 @@ extern "C" int lceq_sweep(lceq_lattice* L, int schedule, int nsweep, int n_or, d
     }
   }
   if ((rc = check_launch())) return rc;
-  if (L->fin_pending) {  // later work on the main stream sees the observables
-    HIPCHK(hipStreamWaitEvent(L->stream, L->ev_fin, 0));
-    L->fin_pending = 0;
-  }
   if (schedule == LCEQ_SWEEP_KP_OR_METRO) {
     HIPCHK(hipMemcpyAsync(delta, L->obs + OBS_DELTA, sizeof(double), hipMemcpyDeviceToHost,
                           L->stream));

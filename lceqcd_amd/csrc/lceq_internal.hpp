@@ -35,12 +35,6 @@ struct lceq_lattice {
   int vol;
   int device;
   hipStream_t stream;
-  // side stream for the per-sweep finalize of production sweeps, ordered by
-  // events: it waits for the Higgs launches; the next sweep's measured link
-  // sweep (which rewrites the partials) waits for it
-  hipStream_t side;
-  hipEvent_t ev_higgs, ev_fin;
-  int fin_pending;
   // device fields
   double *U, *A, *Uf, *Af, *Vst, *F;
   double *partials;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
@@ -82,7 +76,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
 // 0 none, 1 KP (agauge), 2 overrelaxation; higgs_kind 0 none, 1-3 ahiggs1-3;
 // with hist_row the observables are also copied into that history row.
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
-                     int higgs_meas, double* hist_row, hipStream_t stream = nullptr);
+                     int higgs_meas, double* hist_row);
 void launch_staples(lceq_lattice* L, int eo, int mu);
 void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,

@@ -286,7 +286,13 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
       size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
                           : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
       const double* x = p.part + row * p.pstride;
-      for (int i = lane; i < p.nb; i += 64) t += x[i];
+      for (int i0 = lane; i0 < p.nb; i0 += 64 * 8) {
+        double y[8];  // eight loads in flight before the adds
+#pragma unroll
+        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < p.nb ? x[i0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t += y[u];
+      }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     }
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
 }
 
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
-                     int higgs_meas, double* hist_row, hipStream_t stream) {
+                     int higgs_meas, double* hist_row) {
   FinalizeArgs a;
   a.part = L->partials;
   a.pstride = L->pstride;
@@ -350,7 +356,7 @@ void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_ki
   a.vol = (double)L->vol;
   a.obs = L->obs;
   a.hist_row = hist_row;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_TB), 0, stream ? stream : L->stream, a);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_TB), 0, L->stream, a);
 }
 
 // staple only, for the staples() entry point (f/staples.f:9)
@@ -708,7 +714,13 @@ __global__ __launch_bounds__(256) void k_gfix_finalize(const double* __restrict_
                                                        double max_iter, double* __restrict__ obs) {
   if (guarded && obs[OBS_GF_DONE] != 0.0) return;
   double v[1] = {0.0};
-  for (int i = threadIdx.x; i < nb; i += TB) v[0] += part[i];
+  for (int i0 = threadIdx.x; i0 < nb; i0 += TB * 4) {
+    double y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) y[u] = i0 + TB * u < nb ? part[i0 + TB * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] += y[u];
+  }
   block_sum<1>(v);
   if (threadIdx.x == 0) {
     double da = v[0] / n;

@@ -13,6 +13,9 @@ module lceq_c
   integer(c_int), parameter :: LCEQ_OK = 0
   integer(c_int), parameter :: LCEQ_SWEEP_KP_OR_METRO = 0, LCEQ_SWEEP_REFERENCE = 1
 
+  ! lattice handle of the drop-in routines (lceq_dropin.f)
+  type(c_ptr), save :: lceq_handle = c_null_ptr
+
   interface
     integer(c_int) function lceq_create(lx, ly, lz, h) bind(C, name="lceq_create")
       import :: c_int, c_ptr
@@ -67,61 +70,61 @@ module lceq_c
     integer(c_int) function lceq_put_links(h, u, ld) bind(C, name="lceq_put_links")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double), intent(in) :: u(*)
+      type(*), intent(in) :: u(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_links(h, u, ld) bind(C, name="lceq_get_links")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: u(*)
+      type(*) :: u(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_put_higgs(h, a, ld) bind(C, name="lceq_put_higgs")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double), intent(in) :: a(*)
+      type(*), intent(in) :: a(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_higgs(h, a, ld) bind(C, name="lceq_get_higgs")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: a(*)
+      type(*) :: a(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_put_links_fixed(h, u, ld) bind(C, name="lceq_put_links_fixed")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double), intent(in) :: u(*)
+      type(*), intent(in) :: u(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_links_fixed(h, u, ld) bind(C, name="lceq_get_links_fixed")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: u(*)
+      type(*) :: u(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_put_higgs_fixed(h, a, ld) bind(C, name="lceq_put_higgs_fixed")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double), intent(in) :: a(*)
+      type(*), intent(in) :: a(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_higgs_fixed(h, a, ld) bind(C, name="lceq_get_higgs_fixed")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: a(*)
+      type(*) :: a(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_staple(h, v, ld) bind(C, name="lceq_get_staple")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: v(*)
+      type(*) :: v(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_get_force(h, f, ld) bind(C, name="lceq_get_force")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: f(*)
+      type(*) :: f(*)
       integer(c_int64_t), value :: ld
     end function
     integer(c_int) function lceq_unitconf(h) bind(C, name="lceq_unitconf")

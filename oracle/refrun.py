@@ -27,6 +27,9 @@ def available(prec=64):
     return os.access(harness_path(prec), os.X_OK)
 
 
+DROPIN_HARNESS = os.path.join(REF_DIR, "dropin_harness64")
+
+
 def build():
     subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "ref")],
                    check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -34,8 +37,9 @@ def build():
 
 def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
         beta=9.0, beta_a=0.0, beta_2=1.0, beta_4=0.0, lam=1.0, alpha=1.0,
-        limit_or=1e-9, delta=0.4, ex1=None, ex2=None, prec=64):
-    """Run one harness command; returns the list of output records."""
+        limit_or=1e-9, delta=0.4, ex1=None, ex2=None, prec=64, exe=None, env=None):
+    """Run one harness command; returns the list of output records.  exe
+    selects another build of the harness (e.g. the drop-in one)."""
     lx, ly, lz = shape
     n = lx * ly * lz // 2
     hdr = np.array([lx, ly, lz, CMD[cmd], arg1, arg2, imode, nrep, seed],
@@ -54,9 +58,13 @@ def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
             f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
             f.write(ex1.tobytes())
             f.write(ex2.tobytes())
-        env = dict(os.environ)
-        subprocess.run([harness_path(prec), fin, fout], check=True, env=env,
-                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        e = dict(os.environ)
+        e.update(env or {})
+        r = subprocess.run([exe or harness_path(prec), fin, fout], env=e, timeout=300,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        if r.returncode != 0:
+            raise RuntimeError("harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:],
+                                                              r.stderr[-2000:]))
         raw = open(fout, "rb").read()
     recs = []
     off = 0
