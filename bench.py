@@ -12,6 +12,15 @@ step), with plaquette / <rho> / <rho^2> / <a.F> accumulated every sweep.
     python bench.py [--steps K] [--warmup W] [--lattice 64,64,64] [--cpu-seconds S]
 
 Prints one JSON line.  value = site-updates/s (V sites per sweep).
+
+    python bench.py --workload replicas96 [--steps K] [--hist-out FILE]
+
+measures the batched-replica scan instead: 96 independent lattices cycling
+through 16^3, 24^3, 32^3, 40^3, 48^3 and 24^2x96, on the grid
+beta_A in linspace(0.28, 0.40, 12) x beta_2 in linspace(-1.2, -0.6, 8)
+(trace convention, beta_G = 9, beta_4 = 0.1), seeds 12345 + r, all swept by
+the same launches (one replayed hipGraph per sweep); value = aggregate
+site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
 """
 import argparse
 import json
@@ -62,6 +71,68 @@ def cpu_baseline(shape, seconds, n_or):
                       "oracle/lceq_oracle.c single-threaded" % (sweeps, n_or, *shape, el)}
 
 
+REPLICA_SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 32), (40, 40, 40), (48, 48, 48),
+                  (24, 24, 96)]
+
+
+def replicas96(args):
+    import numpy as np
+    import lceqcd_amd
+    from lceqcd_amd import couplings_from_trace_convention
+    nrep = 96
+    shapes = [REPLICA_SHAPES[r % len(REPLICA_SHAPES)] for r in range(nrep)]
+    ba = np.linspace(0.28, 0.40, 12)
+    b2 = np.linspace(-1.2, -0.6, 8)
+    coup = [couplings_from_trace_convention(9.0, ba[r % 12], b2[r // 12], 0.1) for r in range(nrep)]
+    seeds = [12345 + r for r in range(nrep)]
+    B = lceqcd_amd.Batch(shapes)
+    B.set_couplings(coup)
+    B.hotstart(seeds, 0.5)
+    B.rng_philox(seeds)
+    B.sweep(args.warmup, n_or=args.n_or, delta=0.4, measflag=0)
+    B.sync()
+    t0 = time.perf_counter()
+    B.sweep(args.steps, n_or=args.n_or, measflag=1)
+    B.sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    V = B.total_sites
+    hist = B.history(min(args.steps, 1024))
+    tra2 = hist[:, :, 2] / 2.0  # Tr A^2 = |a|^2 / 2 per site (A = a.sigma/2)
+    rec = {
+        "metric": METRIC,
+        "value": V / (ms * 1e-3),
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "96 replicas (16^3..48^3, 24^2x96) on a beta_A x beta_2 grid, 1 KP + %d OR "
+                        "+ ahiggs1 per sweep, one hipGraph per sweep" % args.n_or,
+            "total_sites": V,
+            "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
+            "sweep_algorithmic_frac_of_hbm": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9
+            / HBM_PEAK_GBPS,
+            "tr_a2_mean_range": [float(tra2.mean(0).min()), float(tra2.mean(0).max())],
+        },
+    }
+    if args.hist_out:
+        edges = np.linspace(0, max(1e-9, float(tra2.max())), 41)
+        with open(args.hist_out, "w") as f:
+            json.dump({"edges": edges.tolist(), "replicas": [
+                {"shape": shapes[r], "beta_a": ba[r % 12], "beta_2": b2[r // 12],
+                 "hist": np.histogram(tra2[:, r], edges)[0].tolist(),
+                 "mean": float(tra2[:, r].mean()), "std": float(tra2[:, r].std())}
+                for r in range(nrep)]}, f)
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,11 +142,15 @@ def main():
     ap.add_argument("--n-or", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="single64", choices=["single64", "replicas96"])
+    ap.add_argument("--hist-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(1)
+    if args.workload == "replicas96":
+        return replicas96(args)
 
     import lceqcd_amd
     shape = tuple(int(x) for x in args.lattice.split(","))

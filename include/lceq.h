@@ -171,6 +171,39 @@ int lceq_history(lceq_lattice *L, double *out, int n);
 int lceq_profile_begin(lceq_lattice *L, int kernel_class, int max_launches);
 int lceq_profile_end(lceq_lattice *L, double *total_ms, int *launches);
 
+/* ---- batched replicas ---------------------------------------------------- */
+/* Many independent lattices of uneven sizes and couplings (a phase-diagram
+ * scan, one Markov chain each) swept together: every (parity, mu) link
+ * launch and every Higgs launch covers all replicas, and a sweep is one
+ * replayed hipGraph.  No reference counterpart (the reference runs one
+ * lattice per MPI job).  Replica r follows exactly the chain of a single
+ * lattice with the same seed under lceq_sweep(LCEQ_SWEEP_KP_OR_METRO). */
+typedef struct lceq_batch lceq_batch;
+/* dims[3r..3r+2] = extents of replica r (even, >= 2) */
+int lceq_batch_create(int nrep, const int *dims, lceq_batch **out);
+int lceq_batch_destroy(lceq_batch *B);
+int lceq_batch_info(lceq_batch *B, int *nrep, int64_t *total_sites);
+/* coup[4r..4r+3] = beta, beta_a, beta_2, beta_4 of replica r (reference normalisation) */
+int lceq_batch_set_couplings(lceq_batch *B, const double *coup);
+/* per-replica Philox keys; resets the sweep (pass) counter to 0 */
+int lceq_batch_rng_philox(lceq_batch *B, const uint64_t *seeds);
+int lceq_batch_hotstart(lceq_batch *B, const uint64_t *seeds, double sigma);
+int lceq_batch_put_links(lceq_batch *B, int r, const double *u, int64_t ld);
+int lceq_batch_get_links(lceq_batch *B, int r, double *u, int64_t ld);
+int lceq_batch_put_higgs(lceq_batch *B, int r, const double *a, int64_t ld);
+int lceq_batch_get_higgs(lceq_batch *B, int r, double *a, int64_t ld);
+/* nsweep sweeps of 1 KP + n_or OR + ahiggs1 for every replica; delta (nrep
+ * values, or NULL to keep the device values, initially 0.4) seeds the
+ * ahiggs1 steps; measflag records one history row per replica per sweep;
+ * use_graph replays one captured hipGraph per sweep.  Queued, no sync. */
+int lceq_batch_sweep(lceq_batch *B, int nsweep, int n_or, const double *delta, int measflag,
+                     int use_graph);
+int lceq_batch_sync(lceq_batch *B);
+/* the last n sweeps' rows, out[(k*nrep + r)*16 + i] (i as in lceq_sweep) */
+int lceq_batch_history(lceq_batch *B, double *out, int n);
+/* the current 16 observables of every replica, out[r*16 + i] */
+int lceq_batch_observables(lceq_batch *B, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,20 @@ SIGNATURES = {
     "lceq_history": [VP, DP, I],
     "lceq_profile_begin": [VP, I, I],
     "lceq_profile_end": [VP, DP, IP],
+    "lceq_batch_create": [I, IP, C.POINTER(VP)],
+    "lceq_batch_destroy": [VP],
+    "lceq_batch_info": [VP, IP, C.POINTER(I64)],
+    "lceq_batch_set_couplings": [VP, DP],
+    "lceq_batch_rng_philox": [VP, C.POINTER(U64)],
+    "lceq_batch_hotstart": [VP, C.POINTER(U64), D],
+    "lceq_batch_put_links": [VP, I, DP, I64],
+    "lceq_batch_get_links": [VP, I, DP, I64],
+    "lceq_batch_put_higgs": [VP, I, DP, I64],
+    "lceq_batch_get_higgs": [VP, I, DP, I64],
+    "lceq_batch_sweep": [VP, I, I, DP, I, I],
+    "lceq_batch_sync": [VP],
+    "lceq_batch_history": [VP, DP, I],
+    "lceq_batch_observables": [VP, DP],
 }
 
 _lib = None

@@ -1,0 +1,99 @@
+"""Batched replicas: many independent lattices of uneven sizes and couplings
+swept together by liblceq.so (include/lceq.h, lceq_batch_*).  Replica r
+follows exactly the chain of a single :class:`lceqcd_amd.Lattice` with the
+same seed under ``sweep(SWEEP_KP_OR_METRO, ...)``."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+def _dp(x):
+    return x.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Batch:
+    def __init__(self, shapes):
+        self._L = _lib.load()
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        dims = np.ascontiguousarray(np.array(self.shapes, dtype=np.int32).ravel())
+        h = C.c_void_p()
+        check(self._L.lceq_batch_create(len(self.shapes), dims.ctypes.data_as(C.POINTER(C.c_int)),
+                                        C.byref(h)))
+        self.h = h
+        self.nrep = len(self.shapes)
+        self.sites = [s[0] * s[1] * s[2] for s in self.shapes]
+        self.total_sites = sum(self.sites)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.lceq_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _u64(self, seeds):
+        a = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        assert a.shape == (self.nrep,)
+        return a, a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+    def set_couplings(self, coup):
+        """coup: (nrep, 4) beta, beta_a, beta_2, beta_4 (reference normalisation)."""
+        c = np.ascontiguousarray(np.asarray(coup, dtype=np.float64).reshape(self.nrep, 4))
+        check(self._L.lceq_batch_set_couplings(self.h, _dp(c)))
+
+    def rng_philox(self, seeds):
+        a, p = self._u64(seeds)
+        check(self._L.lceq_batch_rng_philox(self.h, p))
+
+    def hotstart(self, seeds, sigma):
+        a, p = self._u64(seeds)
+        check(self._L.lceq_batch_hotstart(self.h, p, sigma))
+
+    def put_links(self, r, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        check(self._L.lceq_batch_put_links(self.h, r, _dp(u), self.sites[r] // 2))
+
+    def get_links(self, r):
+        n = self.sites[r] // 2
+        u = np.empty((3, 2, n, 4))
+        check(self._L.lceq_batch_get_links(self.h, r, _dp(u), n))
+        return u
+
+    def put_higgs(self, r, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(self._L.lceq_batch_put_higgs(self.h, r, _dp(a), self.sites[r] // 2))
+
+    def get_higgs(self, r):
+        n = self.sites[r] // 2
+        a = np.empty((2, n, 3))
+        check(self._L.lceq_batch_get_higgs(self.h, r, _dp(a), n))
+        return a
+
+    def sweep(self, nsweep, n_or=4, delta=None, measflag=1, graph=True):
+        d = None
+        if delta is not None:
+            d = np.ascontiguousarray(np.broadcast_to(np.asarray(delta, dtype=np.float64),
+                                                     (self.nrep,)))
+        check(self._L.lceq_batch_sweep(self.h, nsweep, n_or, None if d is None else _dp(d),
+                                       measflag, 1 if graph else 0))
+
+    def sync(self):
+        check(self._L.lceq_batch_sync(self.h))
+
+    def history(self, n):
+        """(n, nrep, 16) rows of the last n sweeps (columns as Lattice.history)."""
+        out = np.zeros((n, self.nrep, 16))
+        check(self._L.lceq_batch_history(self.h, _dp(out), n))
+        return out
+
+    def observables(self):
+        out = np.zeros((self.nrep, 16))
+        check(self._L.lceq_batch_observables(self.h, _dp(out)))
+        return out

@@ -17,8 +17,7 @@ using namespace lceq;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int fail(int code, const char* fmt, ...) {
+int lceq::set_error(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -27,6 +26,7 @@ static int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
+#define fail lceq::set_error
 
 #define HIPCHK(call)                                                                     \
   do {                                                                                   \

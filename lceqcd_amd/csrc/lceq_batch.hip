@@ -1,0 +1,502 @@
+// lceq_batch.hip -- many independent lattices of uneven sizes and couplings
+// (a phase-diagram scan) swept together: one launch per (parity, mu) link
+// class and per Higgs parity covers every replica, so the device sees the sum
+// of all replicas' sites per launch.  A sweep's launches are captured once in
+// a hipGraph and replayed; the Philox pass number comes from a device-side
+// sweep counter so that the captured graph stays valid from sweep to sweep.
+//
+// Replica r with seed s_r follows exactly the trajectory that a single
+// lceq_lattice with seed s_r, started at pass 0, follows under
+// lceq_sweep(LCEQ_SWEEP_KP_OR_METRO): the per-site code (lceq_sites.hpp),
+// the Philox counters and the reduction order are the same.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lceq.h"
+#include "lceq_internal.hpp"
+#include "lceq_sites.hpp"
+
+using namespace lceq;
+
+namespace {
+
+constexpr int TB = 256;
+constexpr int FIN_TB = 1024;
+
+struct Replica {
+  Geom g;
+  double *U, *A;
+  double beta, beta_a, beta_2, beta_4;
+  Philox key;
+  int blk0, nb;     // first global block, number of blocks
+  double* part;     // [LCEQ_NSLOT][LCEQ_NQ][pstride]
+  int pstride;
+  double* obs;      // [LCEQ_OBS_N]
+};
+
+struct BatchCommon {
+  const Replica* reps;
+  const int* blk2rep;
+  int total_nb;
+  const uint64_t* sweep;  // device sweep counter
+  int passes_per_sweep;
+};
+
+__device__ __forceinline__ int batch_block(const BatchCommon& c, int& r, const Replica*& R) {
+  int gblk = xcd_block(c.total_nb);
+  r = c.blk2rep[gblk];
+  R = c.reps + r;
+  return gblk - R->blk0;
+}
+
+template <bool KP>
+__global__ __launch_bounds__(256) void k_link_batch(BatchCommon c, int eo, int mu, int pass_in_sweep,
+                                                    int measflag, int reduce) {
+  int r;
+  const Replica* R;
+  int lblk = batch_block(c, r, R);
+  unsigned s = (unsigned)(lblk * TB + threadIdx.x);
+  double red[3] = {0.0, 0.0, 0.0};
+  if (s < (unsigned)R->g.n) {
+    LinkSite p;
+    p.g = R->g;
+    p.U = R->U;
+    p.A = R->A;
+    p.eo = eo;
+    p.mu = mu;
+    p.beta = R->beta;
+    p.beta_a = R->beta_a;
+    p.fed = nullptr;
+    p.key = R->key;
+    p.pass = (*c.sweep) * (uint64_t)c.passes_per_sweep + (uint64_t)pass_in_sweep;
+    p.measflag = measflag;
+    p.renorm = 1;
+    link_update_site<KP, false>(p, s, red);
+  }
+  if (reduce)
+    block_sum_store<3>(red, R->part + (size_t)(eo * 3 + mu) * LCEQ_NQ * R->pstride, R->pstride,
+                       lblk);
+}
+
+__global__ __launch_bounds__(256) void k_higgs_batch(BatchCommon c, int eo, int pass_in_sweep,
+                                                     int measflag) {
+  int r;
+  const Replica* R;
+  int lblk = batch_block(c, r, R);
+  unsigned s = (unsigned)(lblk * TB + threadIdx.x);
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  if (s < (unsigned)R->g.n) {
+    HiggsSite p;
+    p.g = R->g;
+    p.U = R->U;
+    p.A = R->A;
+    p.eo = eo;
+    p.imode = 0;
+    p.beta_a = R->beta_a;
+    p.beta_2 = R->beta_2;
+    p.beta_4 = R->beta_4;
+    p.delta = R->obs + OBS_DELTA;
+    p.fed = nullptr;
+    p.key = R->key;
+    p.pass = (*c.sweep) * (uint64_t)c.passes_per_sweep + (uint64_t)pass_in_sweep;
+    p.measflag = measflag;
+    higgs_update_site<1, false>(p, s, red);
+  }
+  block_sum_store<4>(red, R->part + (size_t)(6 + eo) * LCEQ_NQ * R->pstride, R->pstride, lblk);
+}
+
+// Per-replica close of a sweep (block r): the same reduction and slot order
+// as k_finalize of the single-lattice path, then the history row.
+__global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restrict__ reps,
+                                                         int link_kind, int meas,
+                                                         const uint64_t* __restrict__ sweep,
+                                                         double* __restrict__ hist, int hist_cap,
+                                                         int nrep) {
+  constexpr int NV = 6 * 3 + 2 * 4;
+  __shared__ double tot[32];
+  const Replica* R = reps + blockIdx.x;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    int q = w + 16 * h;
+    if (q >= NV) continue;
+    bool on = q < 18 ? link_kind != 0 : true;
+    double t = 0.0;
+    if (on) {
+      size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
+                          : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
+      const double* x = R->part + row * R->pstride;
+      for (int i0 = lane; i0 < R->nb; i0 += 64 * 8) {
+        double y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < R->nb ? x[i0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t += y[u];
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    }
+    if (lane == 0) tot[q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* v = tot;
+  double* obs = R->obs;
+  double vol = 2.0 * (double)R->g.n;
+  if (link_kind) {
+    double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
+    for (int slot = 0; slot < 6; ++slot) {
+      plaq = plaq + v[slot * 3];
+      nacch += v[slot * 3 + 1];
+      acc1 += v[slot * 3 + 2];
+    }
+    obs[OBS_NACCH] = nacch;
+    obs[OBS_NACC1] = acc1;
+    if (meas == 1) obs[OBS_PLAQ0] = plaq / (12.0 * vol);
+    double accept;
+    if (link_kind == 1) {
+      accept = nacch;
+      if (meas == 1) accept = (accept / (3.0 * vol)) * (acc1 / (12.0 * vol));
+    } else {
+      accept = acc1 / (3.0 * vol);
+    }
+    obs[OBS_ACC_LINK] = accept;
+  }
+  double acc = v[18] + v[22];
+  double r2 = v[19] + v[23], r4 = v[20] + v[24], kin = v[21] + v[25];
+  if (meas == 1) {
+    obs[OBS_A02] = r2 / vol;
+    obs[OBS_A04] = r4 / vol;
+    obs[OBS_A0KIN] = kin / (6.0 * vol);
+  }
+  double accept = acc / vol;
+  obs[OBS_ACC_HIGGS] = accept;
+  double d = obs[OBS_DELTA];
+  if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+  obs[OBS_DELTA] = d;
+  if (meas == 1 && hist) {
+    double* row = hist + ((size_t)(*sweep % (uint64_t)hist_cap) * nrep + blockIdx.x) * LCEQ_OBS_N;
+    for (int i = 0; i < LCEQ_OBS_N; ++i) row[i] = obs[i];
+  }
+}
+
+__global__ void k_tick(uint64_t* sweep) {
+  if (threadIdx.x == 0) *sweep += 1;
+}
+
+}  // namespace
+
+#define bfail lceq::set_error
+
+#define BHIP(call)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return bfail(LCEQ_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
+  } while (0)
+
+struct lceq_batch {
+  int nrep;
+  std::vector<Replica> reps;  // host copy of the device table
+  std::vector<int64_t> field_off;
+  Replica* d_reps;
+  int* d_blk2rep;
+  int total_nb;
+  int64_t total_sites;
+  double *pool, *parts, *obs, *hist;
+  size_t pool_doubles, parts_doubles;
+  int hist_cap;
+  uint64_t* d_sweep;
+  uint64_t sweeps_done;  // host mirror of the device counter
+  hipStream_t stream;
+  hipGraphExec_t graph;
+  int graph_n_or, graph_meas;
+};
+
+static int batch_upload_table(lceq_batch* B) {
+  BHIP(hipMemcpyAsync(B->d_reps, B->reps.data(), B->reps.size() * sizeof(Replica),
+                      hipMemcpyHostToDevice, B->stream));
+  BHIP(hipStreamSynchronize(B->stream));
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
+  if (!out || !dims || nrep < 1) return bfail(LCEQ_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  lceq_batch* B = new lceq_batch();
+  B->nrep = nrep;
+  B->reps.resize(nrep);
+  B->field_off.resize(nrep);
+  int64_t pool = 0, parts = 0;
+  int nb_total = 0;
+  std::vector<int> blk2rep;
+  for (int r = 0; r < nrep; ++r) {
+    int lx = dims[3 * r], ly = dims[3 * r + 1], lz = dims[3 * r + 2];
+    if (lx < 2 || ly < 2 || lz < 2 || (lx & 1) || (ly & 1) || (lz & 1)) {
+      delete B;
+      return bfail(LCEQ_ERR_ARG, "replica %d: extents must be even and >= 2", r);
+    }
+    Replica& R = B->reps[r];
+    memset(&R, 0, sizeof(R));
+    R.g.lx = lx;
+    R.g.ly = ly;
+    R.g.lz = lz;
+    R.g.h1 = lx / 2;
+    R.g.h2 = lx * ly / 2;
+    R.g.n = lx * ly * lz / 2;
+    R.g.npad = (R.g.n + 63) & ~63;
+    R.nb = (R.g.n + TB - 1) / TB;
+    R.blk0 = nb_total;
+    R.pstride = R.nb;
+    R.beta = 9.0;
+    B->field_off[r] = pool;
+    pool += 30 * (int64_t)R.g.npad;  // 24 link + 6 Higgs components
+    parts += (int64_t)LCEQ_NSLOT * LCEQ_NQ * R.pstride;
+    for (int k = 0; k < R.nb; ++k) blk2rep.push_back(r);
+    nb_total += R.nb;
+    B->total_sites += 2 * (int64_t)R.g.n;
+  }
+  B->total_nb = nb_total;
+  B->pool_doubles = pool;
+  B->parts_doubles = parts;
+  B->hist_cap = 1024;
+  int rc = LCEQ_OK;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (rc) return;
+    if (hipMalloc(p, bytes) != hipSuccess) rc = bfail(LCEQ_ERR_HIP, "hipMalloc(%zu) failed", bytes);
+  };
+  if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = bfail(LCEQ_ERR_HIP, "hipStreamCreate failed");
+  alloc((void**)&B->pool, pool * sizeof(double));
+  alloc((void**)&B->parts, parts * sizeof(double));
+  alloc((void**)&B->obs, (size_t)nrep * LCEQ_OBS_N * sizeof(double));
+  alloc((void**)&B->hist, (size_t)B->hist_cap * nrep * LCEQ_OBS_N * sizeof(double));
+  alloc((void**)&B->d_reps, nrep * sizeof(Replica));
+  alloc((void**)&B->d_blk2rep, blk2rep.size() * sizeof(int));
+  alloc((void**)&B->d_sweep, sizeof(uint64_t));
+  if (rc) {
+    lceq_batch_destroy(B);
+    return rc;
+  }
+  int64_t poff = 0;
+  for (int r = 0; r < nrep; ++r) {
+    Replica& R = B->reps[r];
+    R.U = B->pool + B->field_off[r];
+    R.A = R.U + 24 * (size_t)R.g.npad;
+    R.part = B->parts + poff;
+    poff += (int64_t)LCEQ_NSLOT * LCEQ_NQ * R.pstride;
+    R.obs = B->obs + (size_t)r * LCEQ_OBS_N;
+    R.key = Philox{(uint32_t)(12345 + r), 0u};
+  }
+  (void)hipMemsetAsync(B->pool, 0, pool * sizeof(double), B->stream);
+  (void)hipMemsetAsync(B->parts, 0, parts * sizeof(double), B->stream);
+  std::vector<double> obs0((size_t)nrep * LCEQ_OBS_N, 0.0);
+  for (int r = 0; r < nrep; ++r) obs0[(size_t)r * LCEQ_OBS_N + OBS_DELTA] = 0.4;
+  (void)hipMemcpyAsync(B->obs, obs0.data(), obs0.size() * sizeof(double), hipMemcpyHostToDevice,
+                       B->stream);
+  (void)hipMemsetAsync(B->d_sweep, 0, sizeof(uint64_t), B->stream);
+  if (hipMemcpyAsync(B->d_blk2rep, blk2rep.data(), blk2rep.size() * sizeof(int),
+                     hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+      batch_upload_table(B) != LCEQ_OK) {
+    lceq_batch_destroy(B);
+    return bfail(LCEQ_ERR_HIP, "batch initialisation failed");
+  }
+  *out = B;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_destroy(lceq_batch* B) {
+  if (!B) return LCEQ_OK;
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  if (B->graph) (void)hipGraphExecDestroy(B->graph);
+  void* bufs[] = {B->pool, B->parts, B->obs, B->hist, B->d_reps, B->d_blk2rep, B->d_sweep};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_info(lceq_batch* B, int* nrep, int64_t* total_sites) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  if (nrep) *nrep = B->nrep;
+  if (total_sites) *total_sites = B->total_sites;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_set_couplings(lceq_batch* B, const double* coup) {
+  if (!B || !coup) return bfail(LCEQ_ERR_ARG, "null argument");
+  for (int r = 0; r < B->nrep; ++r) {
+    B->reps[r].beta = coup[4 * r];
+    B->reps[r].beta_a = coup[4 * r + 1];
+    B->reps[r].beta_2 = coup[4 * r + 2];
+    B->reps[r].beta_4 = coup[4 * r + 3];
+  }
+  return batch_upload_table(B);
+}
+
+extern "C" int lceq_batch_rng_philox(lceq_batch* B, const uint64_t* seeds) {
+  if (!B || !seeds) return bfail(LCEQ_ERR_ARG, "null argument");
+  for (int r = 0; r < B->nrep; ++r)
+    B->reps[r].key = Philox{(uint32_t)seeds[r], (uint32_t)(seeds[r] >> 32)};
+  uint64_t zero = 0;
+  BHIP(hipMemcpyAsync(B->d_sweep, &zero, sizeof(zero), hipMemcpyHostToDevice, B->stream));
+  B->sweeps_done = 0;
+  return batch_upload_table(B);
+}
+
+extern "C" int lceq_batch_hotstart(lceq_batch* B, const uint64_t* seeds, double sigma) {
+  if (!B || !seeds) return bfail(LCEQ_ERR_ARG, "null argument");
+  // one single-lattice hot start per replica, on a view of the pooled storage
+  for (int r = 0; r < B->nrep; ++r) {
+    lceq_lattice view;
+    memset(&view, 0, sizeof(view));
+    view.g = B->reps[r].g;
+    view.U = B->reps[r].U;
+    view.A = B->reps[r].A;
+    view.stream = B->stream;
+    launch_hotstart(&view, seeds[r], sigma);
+  }
+  BHIP(hipGetLastError());
+  BHIP(hipStreamSynchronize(B->stream));
+  return LCEQ_OK;
+}
+
+static int batch_field(lceq_batch* B, int r, double* host, const double* hin, int64_t ld,
+                       int links) {
+  if (!B || r < 0 || r >= B->nrep) return bfail(LCEQ_ERR_ARG, "bad replica index");
+  const Replica& R = B->reps[r];
+  if (ld < R.g.n) return bfail(LCEQ_ERR_ARG, "ld < N");
+  int ncomp = links ? 4 : 3, nclass = links ? 6 : 2;
+  double* dev = links ? R.U : R.A;
+  std::vector<double> st((size_t)nclass * ncomp * R.g.npad, 0.0);
+  if (hin) {
+    for (int cls = 0; cls < nclass; ++cls)
+      for (int s = 0; s < R.g.n; ++s)
+        for (int c = 0; c < ncomp; ++c)
+          st[((size_t)cls * ncomp + c) * R.g.npad + s] = hin[((size_t)cls * ld + s) * ncomp + c];
+    BHIP(hipMemcpyAsync(dev, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
+                        B->stream));
+    BHIP(hipStreamSynchronize(B->stream));
+  } else {
+    BHIP(hipMemcpyAsync(st.data(), dev, st.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        B->stream));
+    BHIP(hipStreamSynchronize(B->stream));
+    for (int cls = 0; cls < nclass; ++cls)
+      for (int s = 0; s < R.g.n; ++s)
+        for (int c = 0; c < ncomp; ++c)
+          host[((size_t)cls * ld + s) * ncomp + c] = st[((size_t)cls * ncomp + c) * R.g.npad + s];
+  }
+  return LCEQ_OK;
+}
+extern "C" int lceq_batch_put_links(lceq_batch* B, int r, const double* u, int64_t ld) {
+  return batch_field(B, r, nullptr, u, ld, 1);
+}
+extern "C" int lceq_batch_get_links(lceq_batch* B, int r, double* u, int64_t ld) {
+  return batch_field(B, r, u, nullptr, ld, 1);
+}
+extern "C" int lceq_batch_put_higgs(lceq_batch* B, int r, const double* a, int64_t ld) {
+  return batch_field(B, r, nullptr, a, ld, 0);
+}
+extern "C" int lceq_batch_get_higgs(lceq_batch* B, int r, double* a, int64_t ld) {
+  return batch_field(B, r, a, nullptr, ld, 0);
+}
+
+// Enqueue one sweep of every replica: 1 KP + n_or overrelaxation link
+// sweeps + ahiggs1, then the per-replica finalize and the counter tick.
+static void enqueue_sweep(lceq_batch* B, int n_or, int measflag) {
+  BatchCommon c{B->d_reps, B->d_blk2rep, B->total_nb, B->d_sweep, 6 * (1 + n_or) + 2};
+  dim3 grid(B->total_nb), blk(TB);
+  int pass = 0;
+  for (int k = -1; k < n_or; ++k) {
+    int kp = k < 0;
+    int m = measflag && (kp ? n_or == 0 : k == n_or - 1);
+    for (int eo = 0; eo < 2; ++eo)
+      for (int mu = 0; mu < 3; ++mu) {
+        if (kp)
+          hipLaunchKernelGGL(k_link_batch<true>, grid, blk, 0, B->stream, c, eo, mu, pass, m, m);
+        else
+          hipLaunchKernelGGL(k_link_batch<false>, grid, blk, 0, B->stream, c, eo, mu, pass, m, m);
+        ++pass;
+      }
+  }
+  for (int eo = 0; eo < 2; ++eo) {
+    hipLaunchKernelGGL(k_higgs_batch, grid, blk, 0, B->stream, c, eo, pass, measflag);
+    ++pass;
+  }
+  int link_kind = measflag ? (n_or ? 2 : 1) : 0;
+  hipLaunchKernelGGL(k_finalize_batch, dim3(B->nrep), dim3(FIN_TB), 0, B->stream, B->d_reps,
+                     link_kind, measflag, B->d_sweep, B->hist, B->hist_cap, B->nrep);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, B->stream, B->d_sweep);
+}
+
+extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const double* delta,
+                                int measflag, int use_graph) {
+  if (!B || nsweep < 0 || n_or < 0) return bfail(LCEQ_ERR_ARG, "bad sweep arguments");
+  if (delta) {
+    for (int r = 0; r < B->nrep; ++r)
+      BHIP(hipMemcpyAsync(B->obs + (size_t)r * LCEQ_OBS_N + OBS_DELTA, delta + r, sizeof(double),
+                          hipMemcpyHostToDevice, B->stream));
+    BHIP(hipStreamSynchronize(B->stream));
+  }
+  if (!use_graph) {
+    for (int it = 0; it < nsweep; ++it) enqueue_sweep(B, n_or, measflag);
+    BHIP(hipGetLastError());
+  } else {
+    if (!B->graph || B->graph_n_or != n_or || B->graph_meas != measflag) {
+      if (B->graph) {
+        BHIP(hipGraphExecDestroy(B->graph));
+        B->graph = nullptr;
+      }
+      hipGraph_t g;
+      BHIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_sweep(B, n_or, measflag);
+      BHIP(hipStreamEndCapture(B->stream, &g));
+      hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        B->graph = nullptr;
+        return bfail(LCEQ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      }
+      B->graph_n_or = n_or;
+      B->graph_meas = measflag;
+    }
+    for (int it = 0; it < nsweep; ++it) BHIP(hipGraphLaunch(B->graph, B->stream));
+  }
+  B->sweeps_done += nsweep;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_sync(lceq_batch* B) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  BHIP(hipStreamSynchronize(B->stream));
+  return LCEQ_OK;
+}
+
+// rows [sweeps_done - n, sweeps_done) of the per-replica history, as
+// out[(k*nrep + r)*16 + i]; n <= min(sweeps measured, 1024)
+extern "C" int lceq_batch_history(lceq_batch* B, double* out, int n) {
+  if (!B || !out || n < 0 || n > B->hist_cap || (uint64_t)n > B->sweeps_done)
+    return bfail(LCEQ_ERR_ARG, "bad history request");
+  BHIP(hipStreamSynchronize(B->stream));
+  size_t row = (size_t)B->nrep * LCEQ_OBS_N;
+  std::vector<double> h((size_t)B->hist_cap * row);
+  BHIP(hipMemcpy(h.data(), B->hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; ++k) {
+    uint64_t sw = B->sweeps_done - n + k;
+    memcpy(out + k * row, h.data() + (sw % B->hist_cap) * row, row * sizeof(double));
+  }
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_observables(lceq_batch* B, double* out) {
+  if (!B || !out) return bfail(LCEQ_ERR_ARG, "null argument");
+  BHIP(hipStreamSynchronize(B->stream));
+  BHIP(hipMemcpy(out, B->obs, (size_t)B->nrep * LCEQ_OBS_N * sizeof(double),
+                 hipMemcpyDeviceToHost));
+  return LCEQ_OK;
+}

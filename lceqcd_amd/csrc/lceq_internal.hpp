@@ -63,6 +63,9 @@ struct lceq_lattice {
 
 namespace lceq {
 
+// record a failure for lceq_last_error() and return `code`
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
 enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
 
 int blocks_for(int n);

@@ -5,11 +5,11 @@
 #include <math.h>
 
 #include "lceq_internal.hpp"
+#include "lceq_sites.hpp"
 
 namespace lceq {
 
 static constexpr int TB = 256;
-static constexpr double TPI = 2.0 * M_PI;  // 2*acos(-1.0), f/heat_kpi.f:29
 
 int blocks_for(int n) { return (n + TB - 1) / TB; }
 
@@ -39,200 +39,36 @@ __device__ double block_sum_range(const double* __restrict__ x, int n) {
 // ================================================================ links
 
 struct LinkArgs {
-  Geom g;
-  double* U;
-  const double* A;
-  int eo, mu;
-  double beta, beta_a;
-  const double* fed;  // fed uniforms of this (eo, mu) block, reference order
-  Philox key;
-  uint64_t pass;
-  int measflag, renorm;
+  LinkSite site;
   int reduce;
   double* part;  // partial sums of this launch's slot: [LCEQ_NQ][pstride]
   int pstride, nb;
 };
 
-// The 12 links of the staple V(x) of f/staples.f:65-216 for x in parity eo,
-// direction mu, two nu per site: t = 0, 1 for nu = the two directions != mu
-// in increasing order (f/staples.f:48-52).
-struct StapleLinks {
-  su2 mu_xpnu[2];    // U_mu(x+nu)      parity oe  (link 3)
-  su2 nu_x[2];       // U_nu(x)         parity eo  (link 2)
-  su2 nu_xpmu[2];    // U_nu(x+mu)      parity oe  (link 4)
-  su2 mu_xmnu[2];    // U_mu(x-nu)      parity oe  (link 6)
-  su2 nu_xmnu[2];    // U_nu(x-nu)      parity oe  (link 5)
-  su2 nu_xpmumnu[2]; // U_nu(x+mu-nu)   parity eo  (link 7)
-};
-
-__device__ __forceinline__ void load_staple_links(const double* __restrict__ U, const Geom& g,
-                                                  int eo, int mu, unsigned s, Coord c,
-                                                  unsigned sxm, StapleLinks& k) {
-  int oe = eo ^ 1;
-  int mx, my, mz;
-  unit(mu, mx, my, mz);
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    int nu = (t == 0) ? (mu == 0 ? 1 : 0) : (mu == 2 ? 1 : 2);
-    int nx, ny, nz;
-    unit(nu, nx, ny, nz);
-    unsigned sxn = shifted(g, c, nx, ny, nz);
-    unsigned sxmn = shifted(g, c, -nx, -ny, -nz);
-    unsigned sxmmn = shifted(g, c, mx - nx, my - ny, mz - nz);
-    k.mu_xpnu[t] = ld_link(U, g, mu, oe, sxn);
-    k.nu_x[t] = ld_link(U, g, nu, eo, s);
-    k.nu_xpmu[t] = ld_link(U, g, nu, oe, sxm);
-    k.mu_xmnu[t] = ld_link(U, g, mu, oe, sxmn);
-    k.nu_xmnu[t] = ld_link(U, g, nu, oe, sxmn);
-    k.nu_xpmumnu[t] = ld_link(U, g, nu, eo, sxmmn);
-  }
-}
-
-// V = sum_t [ 4 3^+ 2^+ + 7^+ 6^+ 5 ] in the accumulation order of f/staples.f:210-216
-__device__ __forceinline__ su2 staple_sum(const StapleLinks& k) {
-  su2 V{c_mk(0.0, 0.0), c_mk(0.0, 0.0)};
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    su2 up1 = m_dd(k.mu_xpnu[t], k.nu_x[t]);
-    su2 up2 = m_nn(k.nu_xpmu[t], up1);
-    su2 bt1 = m_dn(k.mu_xmnu[t], k.nu_xmnu[t]);
-    su2 bt2 = m_dn(k.nu_xpmumnu[t], bt1);
-    V.a = c_add(c_add(V.a, up2.a), bt2.a);
-    V.b = c_add(c_add(V.b, up2.b), bt2.b);
-  }
-  return V;
-}
-
-__device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& g, int eo, int mu,
-                                      unsigned s, Coord c) {
-  int mx, my, mz;
-  unit(mu, mx, my, mz);
-  StapleLinks k;
-  load_staple_links(U, g, eo, mu, s, c, shifted(g, c, mx, my, mz), k);
-  return staple_sum(k);
-}
-
-__device__ __forceinline__ double hop_action(double beta_a, const vec3& ax, const vec3& axp,
-                                             su2 u) {
-  return beta_a * m_nn(m_an(ax, u), m_ad(axp, u)).a.re;  // f/agauge.f:126-159
-}
-
-__device__ __forceinline__ su2 normalized(su2 u) {
-  double t = (u.a.re * u.a.re + u.a.im * u.a.im) + (u.b.re * u.b.re + u.b.im * u.b.im);
-  t = 1.0 / sqrt(t);
-  return su2{c_scale(t, u.a), c_scale(t, u.b)};
-}
-
-// One (eo, mu) class of f/agauge.f:53-193 (KP=true), or the link
-// overrelaxation U -> Vhat^+ U^+ Vhat^+ with the same hopping Metropolis
-// (KP=false).  Partials: 0 plaquette, 1 KP accepts, 2 Metropolis accepts.
-// All 58 operand loads (and the fed uniforms) are issued before any
-// arithmetic so that one memory latency covers the whole site; the Philox
-// rounds run while the loads are in flight.
 template <bool KP, bool FED>
-__global__ __launch_bounds__(256) void k_link(LinkArgs p) {
-  const Geom& g = p.g;
-  int blk = xcd_block(p.nb);
+__global__ __launch_bounds__(256) void k_link(LinkArgs a) {
+  int blk = xcd_block(a.nb);
   unsigned s = (unsigned)(blk * TB + threadIdx.x);
   double red[3] = {0.0, 0.0, 0.0};
-  if (s < (unsigned)g.n) {
-    int eo = p.eo, oe = eo ^ 1, mu = p.mu;
-    Coord c = site_coord(g, eo, (int)s);
-    int mx, my, mz;
-    unit(mu, mx, my, mz);
-    unsigned sxm = shifted(g, c, mx, my, mz);
-    // ---- load phase
-    StapleLinks k;
-    load_staple_links(p.U, g, eo, mu, s, c, sxm, k);
-    su2 u = ld_link(p.U, g, mu, eo, s);
-    vec3 ax = ld_higgs(p.A, g, eo, s), axp = ld_higgs(p.A, g, oe, sxm);
-    double r1, r2, r3, r4, r5, r6, racc;
-    if (FED) {
-      const double* f = p.fed;
-      unsigned n = (unsigned)g.n;
-      if (KP) {
-        r1 = f[3 * s];
-        r2 = f[3 * s + 1];
-        r3 = f[3 * s + 2];
-        r4 = f[3 * n + s];
-        r5 = f[4 * n + s];
-        r6 = f[5 * n + s];
-        racc = f[6 * n + s];
-      } else {
-        racc = f[s];
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (!FED) {
-      double dummy;
-      if (KP) {
-        philox_pair(p.key, s, 0, p.pass, 0, r1, r2);
-        philox_pair(p.key, s, 0, p.pass, 1, r3, r4);
-        philox_pair(p.key, s, 0, p.pass, 2, r5, r6);
-        philox_pair(p.key, s, 0, p.pass, 3, racc, dummy);
-      } else {
-        philox_pair(p.key, s, 0, p.pass, 0, racc, dummy);
-      }
-    }
-    // ---- compute
-    su2 V = staple_sum(k);
-    // f/agauge.f:71-84
-    double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
-    t = 1.0 / sqrt(t);
-    su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
-    su2 h;
-    if (KP) {
-      double b = t / p.beta;
-      double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
-      // Kennedy-Pendleton, f/heat_kpi.f:42-98
-      if (r1 == 0.0) r1 = 0.0000033;
-      if (r2 == 0.0) r2 = 0.0000066;
-      double cc = cos(TPI * r3);
-      double a0 = -(log(r1) + log(r2) * (cc * cc)) * b;
-      bool lacc = r4 * r4 <= (1.0 - 0.5 * a0);
-      red[1] = lacc ? 1.0 : 0.0;
-      a0 = lacc ? 1.0 - a0 : a0old;
-      double a1 = 1.0 - a0 * a0;
-      a1 = a1 >= 0.0 ? sqrt(a1) : 0.0;
-      double a2 = 2.0 * (r5 - 0.5);
-      su2 w;
-      w.a = c_mk(a0, a1 * a2);
-      double tt = 1.0 - a2 * a2;
-      double a0b = tt >= 0.0 ? a1 * sqrt(tt) : 0.0;
-      double ang = TPI * r6;
-      w.b = c_mk(a0b * sin(ang), a0b * cos(ang));
-      h = m_nd(w, vh);  // f/agauge.f:98
-    } else {
-      su2 r = m_nn(m_nn(vh, u), vh);
-      h = su2{c_conj(r.a), c_neg(r.b)};
-    }
-    double sa_o = hop_action(p.beta_a, ax, axp, u);
-    double sa_n = hop_action(p.beta_a, ax, axp, h);
-    if (exp(sa_o - sa_n) >= racc) {  // f/agauge.f:167-174
-      u = p.renorm ? normalized(h) : h;
-      st_link(p.U, g, mu, eo, s, u);
-      red[2] = 1.0;
-    }
-    if (p.measflag == 1) red[0] = m_nn(u, V).a.re;  // f/agauge.f:179-185
-  }
-  if (p.reduce) block_sum_store<3>(red, p.part, p.pstride, blk);
+  if (s < (unsigned)a.site.g.n) link_update_site<KP, FED>(a.site, s, red);
+  if (a.reduce) block_sum_store<3>(red, a.part, a.pstride, blk);
 }
 
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
                  const double* fed, uint64_t pass, int reduce) {
   LinkArgs a;
-  a.g = L->g;
-  a.U = L->U;
-  a.A = L->A;
-  a.eo = eo;
-  a.mu = mu;
-  a.beta = L->beta;
-  a.beta_a = L->beta_a;
-  a.fed = fed;
-  a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
-  a.pass = pass;
-  a.measflag = measflag;
-  a.renorm = renorm_on_write;
+  a.site.g = L->g;
+  a.site.U = L->U;
+  a.site.A = L->A;
+  a.site.eo = eo;
+  a.site.mu = mu;
+  a.site.beta = L->beta;
+  a.site.beta_a = L->beta_a;
+  a.site.fed = fed;
+  a.site.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
+  a.site.pass = pass;
+  a.site.measflag = measflag;
+  a.site.renorm = renorm_on_write;
   a.reduce = reduce;
   a.part = L->partials + (size_t)(eo * 3 + mu) * LCEQ_NQ * L->pstride;
   a.pstride = L->pstride;
@@ -375,52 +211,6 @@ void launch_staples(lceq_lattice* L, int eo, int mu) {
 
 // ================================================================ Higgs
 
-// Operands of the hopping force F(x) of f/a_force.f:54-110 for x in parity eo
-struct ForceLinks {
-  su2 ux[3];   // U_mu(x)        parity eo
-  su2 uy[3];   // U_mu(x-mu)     parity oe
-  vec3 axp[3]; // a(x+mu)        parity oe
-  vec3 axd[3]; // a(x-mu)        parity oe
-};
-
-__device__ __forceinline__ void load_force_links(const double* __restrict__ U,
-                                                 const double* __restrict__ A, const Geom& g,
-                                                 int eo, unsigned s, Coord c, ForceLinks& k) {
-  int oe = eo ^ 1;
-#pragma unroll
-  for (int mu = 0; mu < 3; ++mu) {
-    int mx, my, mz;
-    unit(mu, mx, my, mz);
-    unsigned sxp = shifted(g, c, mx, my, mz), sxd = shifted(g, c, -mx, -my, -mz);
-    k.ux[mu] = ld_link(U, g, mu, eo, s);
-    k.uy[mu] = ld_link(U, g, mu, oe, sxd);
-    k.axp[mu] = ld_higgs(A, g, oe, sxp);
-    k.axd[mu] = ld_higgs(A, g, oe, sxd);
-  }
-}
-
-// F = sum_mu extract[U(x) X(a(x+mu)) U(x)^+ + U(x-mu)^+ X(a(x-mu)) U(x-mu)]
-__device__ __forceinline__ vec3 force_sum(const ForceLinks& k) {
-  vec3 F{{0.0, 0.0, 0.0}};
-#pragma unroll
-  for (int mu = 0; mu < 3; ++mu) {
-    su2 t1 = m_nn(k.ux[mu], m_ad(k.axp[mu], k.ux[mu]));
-    su2 t2 = m_dn(k.uy[mu], m_an(k.axd[mu], k.uy[mu]));
-    F.x[2] = F.x[2] + c_add(t1.a, t2.a).im;
-    F.x[1] = F.x[1] + c_add(t1.b, t2.b).re;
-    F.x[0] = F.x[0] + c_add(t1.b, t2.b).im;
-  }
-  return F;
-}
-
-__device__ __forceinline__ vec3 hop_force(const double* __restrict__ U,
-                                          const double* __restrict__ A, const Geom& g, int eo,
-                                          unsigned s, Coord c) {
-  ForceLinks k;
-  load_force_links(U, A, g, eo, s, c, k);
-  return force_sum(k);
-}
-
 __global__ __launch_bounds__(256) void k_force(Geom g, const double* __restrict__ U,
                                                const double* __restrict__ A,
                                                double* __restrict__ F, int eo) {
@@ -438,159 +228,36 @@ void launch_force(lceq_lattice* L, int eo) {
 }
 
 struct HiggsArgs {
-  Geom g;
-  const double* U;
-  double* A;
-  int eo, imode;
-  double beta_a, beta_2, beta_4;
-  const double* delta;  // device scalar (ahiggs1)
-  const double* fed;
-  Philox key;
-  uint64_t pass;
-  int measflag;
+  HiggsSite site;
   double* part;
   int pstride, nb;
 };
 
-// KIND 1: f/ahiggs1.f Metropolis; 2: f/ahiggs2.f heatbath; 3: f/ahiggs3.f
-// overrelaxation.  Partials: 0 accepts, 1 rho, 2 rho^2, 3 a.F (post update).
 template <int KIND, bool FED>
-__global__ __launch_bounds__(256) void k_higgs(HiggsArgs p) {
-  const Geom& g = p.g;
-  int blk = xcd_block(p.nb);
+__global__ __launch_bounds__(256) void k_higgs(HiggsArgs a) {
+  int blk = xcd_block(a.nb);
   unsigned s = (unsigned)(blk * TB + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  if (s < (unsigned)g.n) {
-    int eo = p.eo;
-    unsigned n = (unsigned)g.n;
-    Coord c = site_coord(g, eo, (int)s);
-    // ---- load phase: 6 links, 7 Higgs triples and the fed uniforms
-    ForceLinks k;
-    load_force_links(p.U, p.A, g, eo, s, c, k);
-    vec3 a = ld_higgs(p.A, g, eo, s), an;
-    double fu[7];
-    if (FED) {
-      if (KIND == 1) {
-        fu[0] = p.fed[3 * s];
-        fu[1] = p.fed[3 * s + 1];
-        fu[2] = p.fed[3 * s + 2];
-        fu[3] = p.fed[3 * n + s];
-      } else if (KIND == 2) {
-#pragma unroll
-        for (int q = 0; q < 7; ++q) fu[q] = p.fed[(size_t)q * n + s];
-      } else {
-        fu[0] = p.fed[s];
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    vec3 F = force_sum(k);
-    double ro = 0.0, rn = 0.0, boltz, racc;
-    if (KIND == 1) {  // f/ahiggs1.f:51-92
-      double delta = *p.delta, r[3];
-      if (FED) {
-        r[0] = fu[0];
-        r[1] = fu[1];
-        r[2] = fu[2];
-        racc = fu[3];
-      } else {
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, r[0], r[1]);
-        double dummy;
-        philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, r[2], racc);
-        (void)dummy;
-      }
-      double sfo = 0.0, sfn = 0.0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        an.x[i] = a.x[i] + (r[i] - 0.5) * delta;
-        sfo = sfo + a.x[i] * F.x[i];
-        sfn = sfn + an.x[i] * F.x[i];
-        ro = ro + a.x[i] * a.x[i];
-        rn = rn + an.x[i] * an.x[i];
-      }
-      boltz = exp(-(p.beta_a * (sfo - sfn)) + p.beta_2 * (ro - rn) +
-                  p.beta_4 * (ro * ro - rn * rn));
-    } else {
-      double alph, width, cx;
-      if (p.imode == 1) {  // f/ahiggs2.f:61-77
-        double conf2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) conf2 = conf2 + p.beta_a * (F.x[i] * F.x[i]);
-        double d = p.beta_2 + p.beta_4 * conf2;
-        alph = (p.beta_4 * conf2) / (2.0 * (d * d));
-        width = p.beta_2 + alph;
-        cx = 1.0 / 2.0 / width;
-      } else {  // f/ahiggs2.f:52-56
-        alph = 0.0;
-        width = p.beta_2;
-        cx = 1.0 / p.beta_2 / 2.0;
-      }
-      if (KIND == 2) {  // gauss x3 (f/gauss.f:18-29), f/ahiggs2.f:83-91
-        double w[6];
-        if (FED) {
-#pragma unroll
-          for (int q = 0; q < 6; ++q) w[q] = fu[q];
-          racc = fu[6];
-        } else {
-          double dummy;
-          philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, w[0], w[1]);
-          philox_pair(p.key, (uint32_t)s, 0, p.pass, 1, w[2], w[3]);
-          philox_pair(p.key, (uint32_t)s, 0, p.pass, 2, w[4], w[5]);
-          philox_pair(p.key, (uint32_t)s, 0, p.pass, 3, racc, dummy);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          double w1 = -1.0 * (1.0 / width) * log(w[2 * i]);
-          double gi = sqrt(w1) * cos(TPI * w[2 * i + 1]);
-          an.x[i] = gi + p.beta_a * F.x[i] * cx;
-        }
-      } else {  // f/ahiggs3.f:83-87
-        if (FED) {
-          racc = fu[0];
-        } else {
-          double dummy;
-          philox_pair(p.key, (uint32_t)s, 0, p.pass, 0, racc, dummy);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) an.x[i] = -a.x[i] + 2.0 * p.beta_a * F.x[i] * cx;
-      }
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        ro = ro + a.x[i] * a.x[i];
-        rn = rn + an.x[i] * an.x[i];
-      }
-      boltz = exp(-(alph * (ro - rn)) + p.beta_4 * (ro * ro - rn * rn));
-    }
-    if (boltz >= racc) {
-      a = an;
-      ro = rn;
-      st_higgs(p.A, g, eo, s, a);
-      red[0] = 1.0;
-    }
-    if (p.measflag == 1) {
-      red[1] = ro;
-      red[2] = ro * ro;
-      red[3] = (a.x[0] * F.x[0] + a.x[1] * F.x[1]) + a.x[2] * F.x[2];
-    }
-  }
-  block_sum_store<4>(red, p.part, p.pstride, blk);
+  if (s < (unsigned)a.site.g.n) higgs_update_site<KIND, FED>(a.site, s, red);
+  block_sum_store<4>(red, a.part, a.pstride, blk);
 }
 
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
                   uint64_t pass) {
   HiggsArgs a;
-  a.g = L->g;
-  a.U = L->U;
-  a.A = L->A;
-  a.eo = eo;
-  a.imode = L->imode;
-  a.beta_a = L->beta_a;
-  a.beta_2 = L->beta_2;
-  a.beta_4 = L->beta_4;
-  a.delta = L->obs + OBS_DELTA;
-  a.fed = fed;
-  a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
-  a.pass = pass;
-  a.measflag = measflag;
+  a.site.g = L->g;
+  a.site.U = L->U;
+  a.site.A = L->A;
+  a.site.eo = eo;
+  a.site.imode = L->imode;
+  a.site.beta_a = L->beta_a;
+  a.site.beta_2 = L->beta_2;
+  a.site.beta_4 = L->beta_4;
+  a.site.delta = L->obs + OBS_DELTA;
+  a.site.fed = fed;
+  a.site.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
+  a.site.pass = pass;
+  a.site.measflag = measflag;
   a.part = L->partials + (size_t)(6 + eo) * LCEQ_NQ * L->pstride;
   a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
