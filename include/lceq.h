@@ -166,8 +166,11 @@ int lceq_sweep(lceq_lattice *L, int schedule, int nsweep, int n_or, double *delt
 int lceq_history(lceq_lattice *L, double *out, int n);
 
 /* ---- timing (HIP events on the lattice stream) -------------------------- */
-/* bracket every launch of one kernel class with a pair of events until
- * lceq_profile_end, which returns their summed duration and count */
+/* time the launches of one kernel class until lceq_profile_end, which
+ * returns their summed duration and count.  One event pair brackets each run
+ * of consecutive launches of the class (e.g. the 6 * n_or overrelaxation
+ * launches of a schedule-0 sweep), so no event sits between them; the
+ * duration of a run includes the dispatch gaps inside it. */
 int lceq_profile_begin(lceq_lattice *L, int kernel_class, int max_launches);
 int lceq_profile_end(lceq_lattice *L, double *total_ms, int *launches);
 
@@ -203,6 +206,32 @@ int lceq_batch_sync(lceq_batch *B);
 int lceq_batch_history(lceq_batch *B, double *out, int n);
 /* the current 16 observables of every replica, out[r*16 + i] */
 int lceq_batch_observables(lceq_batch *B, double *out);
+
+/* ---- RANLUX (host) --------------------------------------------------------
+ * Luscher's luxury subtract-with-borrow generator, single precision, with the
+ * state, restart vectors and luxury levels of the reference's seeding
+ * generator (f/luxury.f).  Pure host code: no device, no lattice. */
+typedef struct lceq_ranlux_state lceq_ranlux_state;
+/* a generator that initialises itself with the defaults (luxury 3, seed
+ * 314159265) at its first lceq_ranlux call, as f/luxury.f:31-66 */
+int lceq_ranlux_create(lceq_ranlux_state **out);
+int lceq_ranlux_destroy(lceq_ranlux_state *R);
+/* RANLUX(RVEC, LENV), f/luxury.f:12-112: lenv uniforms in (0,1) */
+int lceq_ranlux(lceq_ranlux_state *R, float *rvec, int lenv);
+/* RLUXGO(LUX, INT, K1, K2), f/luxury.f:209-303: luxury 0..4 or a p of 24..2000
+ * (lux < 0: default 3), seed ins (0: default), restart after k1 + 10^9 k2
+ * numbers.  LCEQ_ERR_ARG where the reference prints an error (luxury out of
+ * range, negative seed, impossible restart point), after the same fallbacks. */
+int lceq_rluxgo(lceq_ranlux_state *R, int lux, int ins, int k1, int k2);
+/* RLUXUT(ISDEXT), f/luxury.f:174-190: the 25-integer restart vector */
+int lceq_rluxut(lceq_ranlux_state *R, int *isdext);
+/* RLUXIN(ISDEXT), f/luxury.f:120-171: restore from a restart vector */
+int lceq_rluxin(lceq_ranlux_state *R, const int *isdext);
+/* RLUXAT(LOUT, INOUT, K1, K2), f/luxury.f:194-205 (NULL outputs are skipped) */
+int lceq_rluxat(lceq_ranlux_state *R, int *lout, int *inout, int *k1, int *k2);
+/* the 16 integer seeds MPI rank `rank` derives in f/measinit.f:214-216:
+ * RLUXGO(3, 314159 + rank, 0, 0), 16 RANLUX numbers, iseed = int(10^6 u) */
+int lceq_measinit_seeds(int rank, int *iseed);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,14 @@ SIGNATURES = {
     "lceq_batch_sync": [VP],
     "lceq_batch_history": [VP, DP, I],
     "lceq_batch_observables": [VP, DP],
+    "lceq_ranlux_create": [C.POINTER(VP)],
+    "lceq_ranlux_destroy": [VP],
+    "lceq_ranlux": [VP, C.POINTER(C.c_float), I],
+    "lceq_rluxgo": [VP, I, I, I, I],
+    "lceq_rluxut": [VP, IP],
+    "lceq_rluxin": [VP, IP],
+    "lceq_rluxat": [VP, IP, IP, IP, IP],
+    "lceq_measinit_seeds": [I, IP],
 }
 
 _lib = None

@@ -392,12 +392,15 @@ static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write
   const double* fed0;
   int rc = take_uniforms(L, 6 * per, &fed0);
   if (rc) return rc;
+  int kc = kp ? KC_LINK_KP : KC_LINK_OR;
+  prof_begin(L, kc);
   for (int eo = 0; eo < 2; ++eo)
     for (int mu = 0; mu < 3; ++mu) {
       int slot = eo * 3 + mu;
       launch_link(L, kp, eo, mu, measflag, renorm_on_write, fed0 ? fed0 + slot * per : nullptr,
                   next_pass(L), reduce);
     }
+  prof_end(L, kc);
   return check_launch();
 }
 
@@ -452,8 +455,10 @@ static int higgs_sweep(lceq_lattice* L, int kind, int measflag) {
   const double* fed0;
   int rc = take_uniforms(L, 2 * per, &fed0);
   if (rc) return rc;
+  prof_begin(L, KC_HIGGS);
   for (int eo = 0; eo < 2; ++eo)
     launch_higgs(L, kind, eo, measflag, fed0 ? fed0 + eo * per : nullptr, next_pass(L));
+  prof_end(L, KC_HIGGS);
   return check_launch();
 }
 
@@ -667,10 +672,13 @@ extern "C" int lceq_sweep(lceq_lattice* L, int schedule, int nsweep, int n_or, d
       // finalize launch closes the sweep.
       int meas_kp = measflag && n_or == 0;
       if ((rc = link_sweep(L, 1, meas_kp, 1, meas_kp))) return rc;
+      prof_begin(L, KC_LINK_OR);
       for (int k = 0; k < n_or; ++k) {
         int m = measflag && k == n_or - 1;
-        if ((rc = link_sweep(L, 0, m, 1, m))) return rc;
+        if ((rc = link_sweep(L, 0, m, 1, m))) break;
       }
+      prof_end(L, KC_LINK_OR);
+      if (rc) return rc;
       if ((rc = higgs_sweep(L, 1, measflag))) return rc;
       launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 1, measflag, row);
     } else {  // f/main.f:38-41
@@ -725,6 +733,8 @@ extern "C" int lceq_profile_begin(lceq_lattice* L, int kernel_class, int max_lau
     }
   }
   L->prof_n = 0;
+  L->prof_depth = 0;
+  L->prof_launches = 0;
   L->prof_class = kernel_class;
   return LCEQ_OK;
 }
@@ -741,7 +751,8 @@ extern "C" int lceq_profile_end(lceq_lattice* L, double* total_ms, int* launches
   }
   L->prof_class = 0;
   L->prof_n = 0;
+  L->prof_depth = 0;
   if (total_ms) *total_ms = tot;
-  if (launches) *launches = pairs;
+  if (launches) *launches = L->prof_launches;
   return LCEQ_OK;
 }

@@ -55,9 +55,10 @@ struct lceq_lattice {
   // RNG
   int rng_mode;  // 0 fed, 1 philox
   uint64_t seed, pass;
-  // per-launch timing of one kernel class (bench instrumentation)
+  // timing of one kernel class (bench instrumentation): one event pair per
+  // run of consecutive launches of that class, launches counted inside runs
   int prof_class;
-  int prof_n, prof_cap;
+  int prof_n, prof_cap, prof_depth, prof_launches;
   hipEvent_t *prof_ev;
 };
 
@@ -69,6 +70,26 @@ int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3
 enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
 
 int blocks_for(int n);
+
+// Event brackets around runs of launches of the profiled class.  Brackets
+// nest (a sweep brackets its launches, each launcher brackets itself); only
+// the outermost records events, so back-to-back launches of one class are
+// timed without events between them and the average per launch matches the
+// kernel's own duration up to the dispatch gaps inside the run.
+inline void prof_begin(lceq_lattice* L, int kc) {
+  if (L->prof_class != kc) return;
+  if (L->prof_depth++ == 0 && L->prof_n + 2 <= L->prof_cap)
+    (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
+}
+inline void prof_end(lceq_lattice* L, int kc) {
+  if (L->prof_class != kc) return;
+  if (--L->prof_depth == 0 && (L->prof_n & 1))
+    (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
+}
+// one launch of class kc inside an open run
+inline void prof_count(lceq_lattice* L, int kc) {
+  if (L->prof_class == kc && (L->prof_n & 1)) ++L->prof_launches;
+}
 
 // launchers (lceq_kernels.hip); all enqueue on L->stream, no host sync.
 // Link launches use reduction slots 0-5 ((eo, mu) order), Higgs launches

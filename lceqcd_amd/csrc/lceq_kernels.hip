@@ -13,14 +13,6 @@ static constexpr int TB = 256;
 
 int blocks_for(int n) { return (n + TB - 1) / TB; }
 
-static inline void prof_begin(lceq_lattice* L, int kc) {
-  if (L->prof_class == kc && L->prof_n + 2 <= L->prof_cap)
-    (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
-}
-static inline void prof_end(lceq_lattice* L, int kc) {
-  if (L->prof_class == kc && (L->prof_n & 1)) (void)hipEventRecord(L->prof_ev[L->prof_n++], L->stream);
-}
-
 // deterministic sum of x[0..n) by one 256-thread block; result in all threads
 __device__ double block_sum_range(const double* __restrict__ x, int n) {
   __shared__ double red[4];
@@ -75,6 +67,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.nb = blocks_for(L->g.n);
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
+  prof_count(L, kc);
   if (kp) {
     if (fed)
       hipLaunchKernelGGL((k_link<true, true>), dim3(a.nb), dim3(TB), 0, L->stream, a);
@@ -263,6 +256,7 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.nb = blocks_for(L->g.n);
   dim3 grid(a.nb), blk(TB);
   prof_begin(L, KC_HIGGS);
+  prof_count(L, KC_HIGGS);
 #define LCEQ_HIGGS_LAUNCH(K)                                                         \
   if (fed)                                                                           \
     hipLaunchKernelGGL((k_higgs<K, true>), grid, blk, 0, L->stream, a);              \
@@ -421,6 +415,7 @@ void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarde
   a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
   prof_begin(L, KC_GFIX);
+  prof_count(L, KC_GFIX);
   hipLaunchKernelGGL(k_gfix, dim3(a.nb), dim3(TB), 0, L->stream, a);
   prof_end(L, KC_GFIX);
 }
