@@ -21,6 +21,12 @@ beta_A in linspace(0.28, 0.40, 12) x beta_2 in linspace(-1.2, -0.6, 8)
 (trace convention, beta_G = 9, beta_4 = 0.1), seeds 12345 + r, all swept by
 the same launches (one replayed hipGraph per sweep); value = aggregate
 site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
+
+    python bench.py --calibrate-copy
+
+copies a 512 MiB device buffer three times (a plain copy kernel of known
+size): rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over it give the factors that
+turn those counters into bytes for the lattice kernels.
 """
 import argparse
 import json
@@ -133,6 +139,21 @@ def replicas96(args):
     print(json.dumps(rec), flush=True)
 
 
+def calibrate_copy():
+    import torch
+    n = 64 * 1024 * 1024  # 64 Mi doubles = 512 MiB per buffer
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"calibration": "device copy", "bytes_per_copy": n * 8, "copies": 3,
+                      "read_plus_write_GBps": 3 * 2 * n * 8 / el / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,11 +165,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64", choices=["single64", "replicas96"])
     ap.add_argument("--hist-out", default=None)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.workload == "replicas96":
         return replicas96(args)
 

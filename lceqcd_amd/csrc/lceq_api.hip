@@ -241,8 +241,8 @@ static int ensure_scratch(lceq_lattice* L, size_t bytes) {
   return LCEQ_OK;
 }
 
-// host reference layout <-> device SoA.  ncomp = 4 (links) or 3 (Higgs),
-// nclass = 6 (links: mu*2+p) or 2 (Higgs: p).
+// host reference layout <-> device layout (field_offset).  ncomp = 4 (links)
+// or 3 (Higgs), nclass = 6 (links: mu*2+p) or 2 (Higgs: p).
 static int put_field(lceq_lattice* L, double* dev, const double* host, int64_t ld, int ncomp,
                      int nclass) {
   NEEDL(L);
@@ -252,7 +252,7 @@ static int put_field(lceq_lattice* L, double* dev, const double* host, int64_t l
   for (int cls = 0; cls < nclass; ++cls)
     for (int s = 0; s < g.n; ++s)
       for (int c = 0; c < ncomp; ++c)
-        st[((size_t)cls * ncomp + c) * g.npad + s] = host[((size_t)cls * ld + s) * ncomp + c];
+        st[field_offset(ncomp, cls, c, s, g.npad)] = host[((size_t)cls * ld + s) * ncomp + c];
   HIPCHK(hipMemcpyAsync(dev, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
                         L->stream));
   HIPCHK(hipStreamSynchronize(L->stream));
@@ -271,7 +271,7 @@ static int get_field(lceq_lattice* L, const double* dev, double* host, int64_t l
   for (int cls = 0; cls < nclass; ++cls)
     for (int s = 0; s < g.n; ++s)
       for (int c = 0; c < ncomp; ++c)
-        host[((size_t)cls * ld + s) * ncomp + c] = st[((size_t)cls * ncomp + c) * g.npad + s];
+        host[((size_t)cls * ld + s) * ncomp + c] = st[field_offset(ncomp, cls, c, s, g.npad)];
   return LCEQ_OK;
 }
 

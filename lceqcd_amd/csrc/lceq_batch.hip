@@ -378,7 +378,7 @@ static int batch_field(lceq_batch* B, int r, double* host, const double* hin, in
     for (int cls = 0; cls < nclass; ++cls)
       for (int s = 0; s < R.g.n; ++s)
         for (int c = 0; c < ncomp; ++c)
-          st[((size_t)cls * ncomp + c) * R.g.npad + s] = hin[((size_t)cls * ld + s) * ncomp + c];
+          st[field_offset(ncomp, cls, c, s, R.g.npad)] = hin[((size_t)cls * ld + s) * ncomp + c];
     BHIP(hipMemcpyAsync(dev, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice,
                         B->stream));
     BHIP(hipStreamSynchronize(B->stream));
@@ -389,7 +389,7 @@ static int batch_field(lceq_batch* B, int r, double* host, const double* hin, in
     for (int cls = 0; cls < nclass; ++cls)
       for (int s = 0; s < R.g.n; ++s)
         for (int c = 0; c < ncomp; ++c)
-          host[((size_t)cls * ld + s) * ncomp + c] = st[((size_t)cls * ncomp + c) * R.g.npad + s];
+          host[((size_t)cls * ld + s) * ncomp + c] = st[field_offset(ncomp, cls, c, s, R.g.npad)];
   }
   return LCEQ_OK;
 }

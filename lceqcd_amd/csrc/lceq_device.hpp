@@ -98,48 +98,57 @@ __device__ __forceinline__ void unit(int mu, int& dx, int& dy, int& dz) {
 
 // ------------------------------------------------------------- storage
 
-// Links: component c of U_mu at parity p, site s at
-//   U[((mu*2 + p)*4 + c)*npad + s],   c = Re alpha, Im alpha, Re beta, Im beta.
-// Higgs: A[(p*3 + i)*npad + s].
-// Component bases are uniform per launch and the site offset is an unsigned
-// 32-bit byte offset, so the loads can use the SGPR-base + 32-bit VGPR-offset
-// form (arrays stay below 4 GiB per component).
+// Links: U_mu at parity p, site s as two 16-byte halves
+//   ((double2*)U)[((mu*2 + p)*2 + h)*npad + s],
+//   h = 0: (Re alpha, Im alpha), h = 1: (Re beta, Im beta),
+// so a (mu, p) class still starts at U + (mu*2 + p)*4*npad.
+// Higgs of parity p at A + p*3*npad: (a1, a2) as one double2 array of npad
+// entries, then a3 as a double array of npad entries.
+// A site costs two 16-byte loads per link and a 16- plus an 8-byte load per
+// Higgs triple: every wave-wide load instruction costs the same texture
+// addressing and data-return cycles whether a lane reads 8 or 16 bytes, so
+// the wide halves halve that work per byte.  Class bases are uniform per
+// launch and the site offset is an unsigned 32-bit byte offset, so the loads
+// use the SGPR-base + 32-bit VGPR-offset form (below 4 GiB per array).
 template <typename T>
 __device__ __forceinline__ T& at(T* base, unsigned s) {
-  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (s << 3));
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + s * (unsigned)sizeof(T));
 }
 template <typename T>
 __device__ __forceinline__ const T& at(const T* base, unsigned s) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (s << 3));
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + s * (unsigned)sizeof(T));
 }
 __device__ __forceinline__ su2 ld_link(const double* __restrict__ U, const Geom& g, int mu,
                                        int p, unsigned s) {
-  const double* q = U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad;
-  const size_t np = (size_t)g.npad;
-  return su2{c_mk(at(q, s), at(q + np, s)), c_mk(at(q + 2 * np, s), at(q + 3 * np, s))};
+  const double2* q = reinterpret_cast<const double2*>(U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad);
+  double2 lo = at(q, s), hi = at(q + g.npad, s);
+  return su2{c_mk(lo.x, lo.y), c_mk(hi.x, hi.y)};
 }
 __device__ __forceinline__ void st_link(double* __restrict__ U, const Geom& g, int mu, int p,
                                         unsigned s, su2 u) {
-  double* q = U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad;
-  const size_t np = (size_t)g.npad;
-  at(q, s) = u.a.re;
-  at(q + np, s) = u.a.im;
-  at(q + 2 * np, s) = u.b.re;
-  at(q + 3 * np, s) = u.b.im;
+  double2* q = reinterpret_cast<double2*>(U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad);
+  at(q, s) = make_double2(u.a.re, u.a.im);
+  at(q + g.npad, s) = make_double2(u.b.re, u.b.im);
 }
 __device__ __forceinline__ vec3 ld_higgs(const double* __restrict__ A, const Geom& g, int p,
                                          unsigned s) {
   const double* q = A + (size_t)(p * 3) * (size_t)g.npad;
-  const size_t np = (size_t)g.npad;
-  return vec3{{at(q, s), at(q + np, s), at(q + 2 * np, s)}};
+  double2 v = at(reinterpret_cast<const double2*>(q), s);
+  return vec3{{v.x, v.y, at(q + 2 * (size_t)g.npad, s)}};
 }
 __device__ __forceinline__ void st_higgs(double* __restrict__ A, const Geom& g, int p, unsigned s,
                                          const vec3& a) {
   double* q = A + (size_t)(p * 3) * (size_t)g.npad;
-  const size_t np = (size_t)g.npad;
-  at(q, s) = a.x[0];
-  at(q + np, s) = a.x[1];
-  at(q + 2 * np, s) = a.x[2];
+  at(reinterpret_cast<double2*>(q), s) = make_double2(a.x[0], a.x[1]);
+  at(q + 2 * (size_t)g.npad, s) = a.x[2];
+}
+
+// Offset in doubles of component c (0..ncomp-1) of site s in class cls of a
+// link (ncomp = 4) or Higgs (ncomp = 3) field in the layout above.
+__host__ __device__ inline size_t field_offset(int ncomp, int cls, int c, size_t s, size_t npad) {
+  size_t base = (size_t)cls * ncomp * npad;
+  if (ncomp == 4) return base + (size_t)(c >> 1) * 2 * npad + 2 * s + (c & 1);
+  return base + (c < 2 ? 2 * s + c : 2 * npad + s);
 }
 
 // ----------------------------------------------------------------- RNG

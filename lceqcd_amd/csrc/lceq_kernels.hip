@@ -209,10 +209,7 @@ __global__ __launch_bounds__(256) void k_force(Geom g, const double* __restrict_
                                                double* __restrict__ F, int eo) {
   int s = blockIdx.x * TB + threadIdx.x;
   if (s >= g.n) return;
-  vec3 f = hop_force(U, A, g, eo, s, site_coord(g, eo, s));
-  F[s] = f.x[0];
-  F[g.npad + s] = f.x[1];
-  F[2 * (size_t)g.npad + s] = f.x[2];
+  st_higgs(F, g, 0, s, hop_force(U, A, g, eo, s, site_coord(g, eo, s)));
 }
 
 void launch_force(lceq_lattice* L, int eo) {
