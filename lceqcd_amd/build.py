@@ -17,6 +17,11 @@ FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # keep the compiler away from packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # every +, -, *, / rounded as written, as in the reference's x86 build:
+    # the fields of the updates without transcendental functions (link
+    # overrelaxation, ahiggs1, ahiggs3) equal the CPU restatement's bit for
+    # bit, and a site's result does not depend on the code inlined around it
+    "-ffp-contract=off",
     "-Wall", "-Wno-unknown-warning-option",
 ]
 

@@ -199,9 +199,11 @@ __device__ __forceinline__ void philox_pair(const Philox& key, uint32_t idx, uin
 // Deterministic in-block sum of NQ doubles (blockDim = 256 = 4 waves of 64):
 // xor-butterfly inside each wave, then the four wave totals in fixed order.
 // Every thread of the block receives the NQ totals.
-template <int NQ>
+// Deterministic block sum over NW waves (fixed butterfly and combine order);
+// the result is left in every thread.
+template <int NQ, int NW = 4>
 __device__ __forceinline__ void block_sum(double (&v)[NQ]) {
-  __shared__ double red[4][NQ];
+  __shared__ double red[NW][NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     double t = v[q];
@@ -216,14 +218,18 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ]) {
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) v[q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  for (int q = 0; q < NQ; ++q) {
+    double t = red[0][q];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) t += red[w][q];
+    v[q] = t;
+  }
   __syncthreads();
 }
-
-template <int NQ>
+template <int NQ, int NW = 4>
 __device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* __restrict__ out,
                                                 int stride, int blk) {
-  block_sum<NQ>(v);
+  block_sum<NQ, NW>(v);
   if (threadIdx.x < NQ) {
     int q = threadIdx.x;
     double t = v[0];

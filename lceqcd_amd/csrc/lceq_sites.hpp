@@ -142,7 +142,22 @@ __device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, 
       philox_pair(p.key, s, 0, p.pass, 0, racc, dummy);
     }
   }
-  // ---- compute
+  // ---- compute.  The parts of the Kennedy-Pendleton step that depend only
+  // on the uniforms come first, so that they run under the load latency; the
+  // operations and their order are those of f/heat_kpi.f:42-98.
+  double kp_q = 0.0, kp_a2 = 0.0, kp_st = 0.0, kp_sin = 0.0, kp_cos = 0.0;
+  if (KP) {
+    if (r1 == 0.0) r1 = 0.0000033;
+    if (r2 == 0.0) r2 = 0.0000066;
+    double cc = cos(TPI * r3);
+    kp_q = log(r1) + log(r2) * (cc * cc);
+    kp_a2 = 2.0 * (r5 - 0.5);
+    double tt = 1.0 - kp_a2 * kp_a2;
+    kp_st = tt >= 0.0 ? sqrt(tt) : -1.0;
+    double ang = TPI * r6;
+    kp_sin = sin(ang);
+    kp_cos = cos(ang);
+  }
   su2 V = staple_sum(k);
   // f/agauge.f:71-84
   double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
@@ -153,22 +168,16 @@ __device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, 
     double b = t / p.beta;
     double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
     // Kennedy-Pendleton, f/heat_kpi.f:42-98
-    if (r1 == 0.0) r1 = 0.0000033;
-    if (r2 == 0.0) r2 = 0.0000066;
-    double cc = cos(TPI * r3);
-    double a0 = -(log(r1) + log(r2) * (cc * cc)) * b;
+    double a0 = -kp_q * b;
     bool lacc = r4 * r4 <= (1.0 - 0.5 * a0);
     red[1] = lacc ? 1.0 : 0.0;
     a0 = lacc ? 1.0 - a0 : a0old;
     double a1 = 1.0 - a0 * a0;
     a1 = a1 >= 0.0 ? sqrt(a1) : 0.0;
-    double a2 = 2.0 * (r5 - 0.5);
     su2 w;
-    w.a = c_mk(a0, a1 * a2);
-    double tt = 1.0 - a2 * a2;
-    double a0b = tt >= 0.0 ? a1 * sqrt(tt) : 0.0;
-    double ang = TPI * r6;
-    w.b = c_mk(a0b * sin(ang), a0b * cos(ang));
+    w.a = c_mk(a0, a1 * kp_a2);
+    double a0b = kp_st >= 0.0 ? a1 * kp_st : 0.0;
+    w.b = c_mk(a0b * kp_sin, a0b * kp_cos);
     h = m_nd(w, vh);  // f/agauge.f:98
   } else {
     su2 r = m_nn(m_nn(vh, u), vh);

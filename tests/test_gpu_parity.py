@@ -3,7 +3,9 @@ vectors and against the CPU restatement, on the same inputs.
 
 Bar: 1e-12 relative in FP64 for fields and observables (tolerances are stated
 per case in tests/parity_cases.py), exact for counts, zero accept/reject flips
-(a flip would show up as an O(1) field difference and fail the bar)."""
+(a flip would show up as an O(1) field difference and fail the bar); bit for
+bit for the updates whose fields involve no transcendental function (link
+overrelaxation, ahiggs1, ahiggs3)."""
 import numpy as np
 import pytest
 
@@ -66,6 +68,32 @@ def test_philox_link_updates(make, omake, shape):
         assert_close(G._pl, O._pl, 1e-12, "plaq0 (OR)")
     assert_close(G.get_links(), O.get_links(), 1e-12, "links")
     assert G.rng_state()[1] == O.rng_state()[1]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
+def test_philox_or_and_metropolis_bitwise(make, omake, shape):
+    """Link overrelaxation, ahiggs1 and ahiggs3 use exp only to decide an
+    accept, and the library is built without floating-point contraction, so
+    their fields equal the oracle's to the bit (only an exp result differing
+    by an ulp exactly at the accept threshold could break this)."""
+    print("bitwise OR/Metropolis", shape, flush=True)
+    G, O = _pair(make, omake, shape, 4, coup=PROD, asig=1.0)
+    for k in range(2):
+        for L in (G, O):
+            L.link_or(1)
+        assert np.array_equal(G.get_links().view(np.int64), O.get_links().view(np.int64))
+    dG = dO = 0.4
+    for k in range(2):
+        dG, _ = G.ahiggs1(dG, 1)
+        dO, _ = O.ahiggs1(dO, 1)
+        assert dG == dO
+        assert np.array_equal(G.get_higgs().view(np.int64), O.get_higgs().view(np.int64))
+    G.ahiggs3(1)
+    O.ahiggs3(1)
+    assert np.array_equal(G.get_higgs().view(np.int64), O.get_higgs().view(np.int64))
+    G.link_or(1)
+    O.link_or(1)
+    assert np.array_equal(G.get_links().view(np.int64), O.get_links().view(np.int64))
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
