@@ -35,6 +35,7 @@ extern "C" {
 #define LCEQ_ERR_HIP (-2)         /* HIP runtime failure */
 #define LCEQ_ERR_RNG (-3)         /* fed uniform stream exhausted */
 #define LCEQ_ERR_NOT_CONVERGED (-4) /* landau_gauge hit max_iter */
+#define LCEQ_ERR_IO (-5)          /* configuration file missing, short or unwritable */
 
 /* uniform sources */
 #define LCEQ_RNG_FED 0
@@ -58,6 +59,8 @@ typedef struct lceq_lattice lceq_lattice;
  * (f/initialize.f:35-45, f/adresses.f:18) for one device. */
 int lceq_create(int lx, int ly, int lz, lceq_lattice **out);
 int lceq_destroy(lceq_lattice *L);
+/* the extents lx, ly, lz of the lattice */
+int lceq_dims(lceq_lattice *L, int *dims);
 const char *lceq_last_error(void);
 int lceq_sync(lceq_lattice *L);
 
@@ -206,6 +209,32 @@ int lceq_batch_sync(lceq_batch *B);
 int lceq_batch_history(lceq_batch *B, double *out, int n);
 /* the current 16 observables of every replica, out[r*16 + i] */
 int lceq_batch_observables(lceq_batch *B, double *out);
+
+/* ---- configuration files (host) -------------------------------------------
+ * The reference's direct-access unformatted configuration files
+ * (f/confsave.f:50-253, f/confget.f:318-526): link records per (mu, parity,
+ * z) of 16*2*Lx*Ly/2 bytes, Higgs records per (parity, z) of 8*3*Lx*Ly/2
+ * bytes, each the xy half-plane in site order x/2 + (Lx/2) y.  real_bytes = 8
+ * for a build with 8-byte reals (records full), 4 for the as-shipped build
+ * (values in the first half of each record, the rest zero on write).  Either
+ * path may be NULL to skip that field.  Host arrays use the lceq_put_links /
+ * lceq_put_higgs layout with ld = N. */
+int lceq_conf_write(const int *dims, const double *u, const double *a, const char *u_path,
+                    const char *a_path, int real_bytes);
+int lceq_conf_read(const int *dims, double *u, double *a, const char *u_path,
+                   const char *a_path, int real_bytes);
+/* CONFSAVE (f/confsave.f:18): download the lattice and write the files */
+int lceq_confsave(lceq_lattice *L, const char *u_path, const char *a_path, int real_bytes);
+/* OLDCONF (f/confget.f:318): read the files and upload them to the lattice */
+int lceq_oldconf(lceq_lattice *L, const char *u_path, const char *a_path, int real_bytes);
+/* The file name f/confsave.f:249-253 writes (upper = 1: '_BE', '_U') or the
+ * one f/measinit.f:246-250 makes oldconf open (upper = 0: '_be', '_u'): the
+ * two differ in case.  higgs selects a_conf_ over u_conf_; beta, x, y and
+ * sweep as the reference passes them (nint(1000 beta), nint(10^4 x),
+ * nint(10^4 y), confstart + nsweep) with Fortran I edit descriptors (a value
+ * too wide for its field becomes asterisks, e.g. a negative y). */
+int lceq_conf_filename(int higgs, int upper, int lz, double beta, double x, double y, int sweep,
+                       char *out, int cap);
 
 /* ---- RANLUX (host) --------------------------------------------------------
  * Luscher's luxury subtract-with-borrow generator, single precision, with the

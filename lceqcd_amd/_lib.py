@@ -68,6 +68,12 @@ SIGNATURES = {
     "lceq_batch_sync": [VP],
     "lceq_batch_history": [VP, DP, I],
     "lceq_batch_observables": [VP, DP],
+    "lceq_dims": [VP, IP],
+    "lceq_conf_write": [IP, DP, DP, C.c_char_p, C.c_char_p, I],
+    "lceq_conf_read": [IP, DP, DP, C.c_char_p, C.c_char_p, I],
+    "lceq_confsave": [VP, C.c_char_p, C.c_char_p, I],
+    "lceq_oldconf": [VP, C.c_char_p, C.c_char_p, I],
+    "lceq_conf_filename": [I, I, I, D, D, D, I, C.c_char_p, I],
     "lceq_ranlux_create": [C.POINTER(VP)],
     "lceq_ranlux_destroy": [VP],
     "lceq_ranlux": [VP, C.POINTER(C.c_float), I],

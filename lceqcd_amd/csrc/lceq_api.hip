@@ -165,6 +165,15 @@ extern "C" int lceq_set_gauge_fixing(lceq_lattice* L, double lambda, double alph
   return LCEQ_OK;
 }
 
+extern "C" int lceq_dims(lceq_lattice* L, int* dims) {
+  NEEDL(L);
+  if (!dims) return fail(LCEQ_ERR_ARG, "null output");
+  dims[0] = L->g.lx;
+  dims[1] = L->g.ly;
+  dims[2] = L->g.lz;
+  return LCEQ_OK;
+}
+
 extern "C" int lceq_set_imode(lceq_lattice* L, int imode) {
   NEEDL(L);
   if (imode != 0 && imode != 1) return fail(LCEQ_ERR_ARG, "imode must be 0 or 1");

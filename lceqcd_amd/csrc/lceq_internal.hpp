@@ -70,6 +70,9 @@ int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3
 enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
 
 int blocks_for(int n);
+// sites per thread of the link launches, and their block count for n sites
+constexpr int LINK_SPT = 2;
+int link_blocks(int n);
 
 // Event brackets around runs of launches of the profiled class.  Brackets
 // nest (a sweep brackets its launches, each launcher brackets itself); only

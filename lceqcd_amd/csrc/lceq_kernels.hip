@@ -37,14 +37,32 @@ struct LinkArgs {
   int pstride, nb;
 };
 
+// LINK_SPT sites per thread, TB * LINK_SPT consecutive sites per block.  A
+// thread issues the loads of all its sites before the first update, so the
+// arithmetic of its first site overlaps the transfer of the later ones.
+// Sites past the end load site 0 and skip the update.
 template <bool KP, bool FED>
 __global__ __launch_bounds__(256) void k_link(LinkArgs a) {
   int blk = xcd_block(a.nb);
-  unsigned s = (unsigned)(blk * TB + threadIdx.x);
+  unsigned n = (unsigned)a.site.g.n;
+  unsigned s[LINK_SPT];
+  LinkOperands o[LINK_SPT];
+#pragma unroll
+  for (int j = 0; j < LINK_SPT; ++j) {
+    s[j] = (unsigned)((blk * LINK_SPT + j) * TB + threadIdx.x);
+    link_load<KP, FED>(a.site, s[j] < n ? s[j] : 0u, o[j]);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int j = 0; j < LINK_SPT; ++j) link_prepare<KP, FED>(a.site, s[j] < n ? s[j] : 0u, o[j]);
   double red[3] = {0.0, 0.0, 0.0};
-  if (s < (unsigned)a.site.g.n) link_update_site<KP, FED>(a.site, s, red);
+#pragma unroll
+  for (int j = 0; j < LINK_SPT; ++j)
+    if (s[j] < n) link_finish<KP>(a.site, s[j], o[j], red);
   if (a.reduce) block_sum_store<3>(red, a.part, a.pstride, blk);
 }
+
+int link_blocks(int n) { return (n + TB * LINK_SPT - 1) / (TB * LINK_SPT); }
 
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
                  const double* fed, uint64_t pass, int reduce) {
@@ -64,7 +82,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.reduce = reduce;
   a.part = L->partials + (size_t)(eo * 3 + mu) * LCEQ_NQ * L->pstride;
   a.pstride = L->pstride;
-  a.nb = blocks_for(L->g.n);
+  a.nb = link_blocks(L->g.n);
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
   prof_count(L, kc);
@@ -86,7 +104,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
 
 struct FinalizeArgs {
   const double* part;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
-  int pstride, nb;
+  int pstride, nb_link, nb_higgs;  // block partials per link / Higgs slot
   int link_kind, link_meas, higgs_kind, higgs_meas;
   double vol;
   double* obs;
@@ -115,10 +133,11 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
       size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
                           : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
       const double* x = p.part + row * p.pstride;
-      for (int i0 = lane; i0 < p.nb; i0 += 64 * 8) {
+      int nb = q < 18 ? p.nb_link : p.nb_higgs;
+      for (int i0 = lane; i0 < nb; i0 += 64 * 8) {
         double y[8];  // eight loads in flight before the adds
 #pragma unroll
-        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < p.nb ? x[i0 + 64 * u] : 0.0;
+        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < nb ? x[i0 + 64 * u] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u) t += y[u];
       }
@@ -177,7 +196,8 @@ void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_ki
   FinalizeArgs a;
   a.part = L->partials;
   a.pstride = L->pstride;
-  a.nb = blocks_for(L->g.n);
+  a.nb_link = link_blocks(L->g.n);
+  a.nb_higgs = blocks_for(L->g.n);
   a.link_kind = link_kind;
   a.link_meas = link_meas;
   a.higgs_kind = higgs_kind;

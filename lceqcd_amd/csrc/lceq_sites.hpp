@@ -95,70 +95,88 @@ __device__ __forceinline__ su2 normalized(su2 u) {
   return su2{c_scale(t, u.a), c_scale(t, u.b)};
 }
 
-// One (eo, mu) class of f/agauge.f:53-193 (KP=true), or the link
-// overrelaxation U -> Vhat^+ U^+ Vhat^+ with the same hopping Metropolis
-// (KP=false).  Partials: 0 plaquette, 1 KP accepts, 2 Metropolis accepts.
-// All 58 operand loads (and the fed uniforms) are issued before any
-// arithmetic so that one memory latency covers the whole site; the Philox
-// rounds run while the loads are in flight.
+// Operands of one link update: the staple links, the link, the two Higgs
+// triples of the hopping term and the uniforms (with the uniform-only part of
+// the Kennedy-Pendleton step).
+struct LinkOperands {
+  StapleLinks k;
+  su2 u;
+  vec3 ax, axp;
+  double r4, racc;                          // KP acceptance and Metropolis uniforms
+  double r1, r2, r3, r5, r6;                // KP uniforms (until link_prepare)
+  double kp_q, kp_a2, kp_st, kp_sin, kp_cos;  // f/heat_kpi.f:42-98, uniform-only part
+};
+
+// Load phase: the 30 operand loads (and the fed uniforms) of site s
 template <bool KP, bool FED>
-__device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, double (&red)[3]) {
+__device__ __forceinline__ void link_load(const LinkSite& p, unsigned s, LinkOperands& o) {
   const Geom& g = p.g;
   int eo = p.eo, oe = eo ^ 1, mu = p.mu;
   Coord c = site_coord(g, eo, (int)s);
   int mx, my, mz;
   unit(mu, mx, my, mz);
   unsigned sxm = shifted(g, c, mx, my, mz);
-  // ---- load phase
-  StapleLinks k;
-  load_staple_links(p.U, g, eo, mu, s, c, sxm, k);
-  su2 u = ld_link(p.U, g, mu, eo, s);
-  vec3 ax = ld_higgs(p.A, g, eo, s), axp = ld_higgs(p.A, g, oe, sxm);
-  double r1, r2, r3, r4, r5, r6, racc;
+  load_staple_links(p.U, g, eo, mu, s, c, sxm, o.k);
+  o.u = ld_link(p.U, g, mu, eo, s);
+  o.ax = ld_higgs(p.A, g, eo, s);
+  o.axp = ld_higgs(p.A, g, oe, sxm);
   if (FED) {
     const double* f = p.fed;
     unsigned n = (unsigned)g.n;
     if (KP) {
-      r1 = f[3 * s];
-      r2 = f[3 * s + 1];
-      r3 = f[3 * s + 2];
-      r4 = f[3 * n + s];
-      r5 = f[4 * n + s];
-      r6 = f[5 * n + s];
-      racc = f[6 * n + s];
+      o.r1 = f[3 * s];
+      o.r2 = f[3 * s + 1];
+      o.r3 = f[3 * s + 2];
+      o.r4 = f[3 * n + s];
+      o.r5 = f[4 * n + s];
+      o.r6 = f[5 * n + s];
+      o.racc = f[6 * n + s];
     } else {
-      racc = f[s];
+      o.racc = f[s];
     }
   }
-  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Uniform phase, independent of the loads: the Philox draws and the part of
+// the Kennedy-Pendleton step that depends only on them (operations and order
+// of f/heat_kpi.f:42-98), so that it runs under the load latency.
+template <bool KP, bool FED>
+__device__ __forceinline__ void link_prepare(const LinkSite& p, unsigned s, LinkOperands& o) {
   if (!FED) {
     double dummy;
     if (KP) {
-      philox_pair(p.key, s, 0, p.pass, 0, r1, r2);
-      philox_pair(p.key, s, 0, p.pass, 1, r3, r4);
-      philox_pair(p.key, s, 0, p.pass, 2, r5, r6);
-      philox_pair(p.key, s, 0, p.pass, 3, racc, dummy);
+      philox_pair(p.key, s, 0, p.pass, 0, o.r1, o.r2);
+      philox_pair(p.key, s, 0, p.pass, 1, o.r3, o.r4);
+      philox_pair(p.key, s, 0, p.pass, 2, o.r5, o.r6);
+      philox_pair(p.key, s, 0, p.pass, 3, o.racc, dummy);
     } else {
-      philox_pair(p.key, s, 0, p.pass, 0, racc, dummy);
+      philox_pair(p.key, s, 0, p.pass, 0, o.racc, dummy);
     }
   }
-  // ---- compute.  The parts of the Kennedy-Pendleton step that depend only
-  // on the uniforms come first, so that they run under the load latency; the
-  // operations and their order are those of f/heat_kpi.f:42-98.
-  double kp_q = 0.0, kp_a2 = 0.0, kp_st = 0.0, kp_sin = 0.0, kp_cos = 0.0;
   if (KP) {
+    double r1 = o.r1, r2 = o.r2;
     if (r1 == 0.0) r1 = 0.0000033;
     if (r2 == 0.0) r2 = 0.0000066;
-    double cc = cos(TPI * r3);
-    kp_q = log(r1) + log(r2) * (cc * cc);
-    kp_a2 = 2.0 * (r5 - 0.5);
-    double tt = 1.0 - kp_a2 * kp_a2;
-    kp_st = tt >= 0.0 ? sqrt(tt) : -1.0;
-    double ang = TPI * r6;
-    kp_sin = sin(ang);
-    kp_cos = cos(ang);
+    double cc = cos(TPI * o.r3);
+    o.kp_q = log(r1) + log(r2) * (cc * cc);
+    o.kp_a2 = 2.0 * (o.r5 - 0.5);
+    double tt = 1.0 - o.kp_a2 * o.kp_a2;
+    o.kp_st = tt >= 0.0 ? sqrt(tt) : -1.0;
+    double ang = TPI * o.r6;
+    o.kp_sin = sin(ang);
+    o.kp_cos = cos(ang);
   }
-  su2 V = staple_sum(k);
+}
+
+// Update phase of one (eo, mu) class of f/agauge.f:53-193 (KP=true), or the
+// link overrelaxation U -> Vhat^+ U^+ Vhat^+ with the same hopping
+// Metropolis (KP=false).  Partials (added to red): 0 plaquette, 1 KP
+// accepts, 2 Metropolis accepts.
+template <bool KP>
+__device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const LinkOperands& o,
+                                            double (&red)[3]) {
+  su2 u = o.u;
+  su2 V = staple_sum(o.k);
   // f/agauge.f:71-84
   double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
   t = 1.0 / sqrt(t);
@@ -168,29 +186,40 @@ __device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, 
     double b = t / p.beta;
     double a0old = c_mul(u.a, vh.a).re - c_mul(u.b, c_conj(vh.b)).re;
     // Kennedy-Pendleton, f/heat_kpi.f:42-98
-    double a0 = -kp_q * b;
-    bool lacc = r4 * r4 <= (1.0 - 0.5 * a0);
-    red[1] = lacc ? 1.0 : 0.0;
+    double a0 = -o.kp_q * b;
+    bool lacc = o.r4 * o.r4 <= (1.0 - 0.5 * a0);
+    red[1] += lacc ? 1.0 : 0.0;
     a0 = lacc ? 1.0 - a0 : a0old;
     double a1 = 1.0 - a0 * a0;
     a1 = a1 >= 0.0 ? sqrt(a1) : 0.0;
     su2 w;
-    w.a = c_mk(a0, a1 * kp_a2);
-    double a0b = kp_st >= 0.0 ? a1 * kp_st : 0.0;
-    w.b = c_mk(a0b * kp_sin, a0b * kp_cos);
+    w.a = c_mk(a0, a1 * o.kp_a2);
+    double a0b = o.kp_st >= 0.0 ? a1 * o.kp_st : 0.0;
+    w.b = c_mk(a0b * o.kp_sin, a0b * o.kp_cos);
     h = m_nd(w, vh);  // f/agauge.f:98
   } else {
     su2 r = m_nn(m_nn(vh, u), vh);
     h = su2{c_conj(r.a), c_neg(r.b)};
   }
-  double sa_o = hop_action(p.beta_a, ax, axp, u);
-  double sa_n = hop_action(p.beta_a, ax, axp, h);
-  if (exp(sa_o - sa_n) >= racc) {  // f/agauge.f:167-174
+  double sa_o = hop_action(p.beta_a, o.ax, o.axp, u);
+  double sa_n = hop_action(p.beta_a, o.ax, o.axp, h);
+  if (exp(sa_o - sa_n) >= o.racc) {  // f/agauge.f:167-174
     u = p.renorm ? normalized(h) : h;
-    st_link(p.U, g, mu, eo, s, u);
-    red[2] = 1.0;
+    st_link(p.U, p.g, p.mu, p.eo, s, u);
+    red[2] += 1.0;
   }
-  if (p.measflag == 1) red[0] = m_nn(u, V).a.re;  // f/agauge.f:179-185
+  if (p.measflag == 1) red[0] += m_nn(u, V).a.re;  // f/agauge.f:179-185
+}
+
+// One site: all operand loads are issued before any arithmetic, so that one
+// memory latency covers the site; the uniform phase runs under it.
+template <bool KP, bool FED>
+__device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, double (&red)[3]) {
+  LinkOperands o;
+  link_load<KP, FED>(p, s, o);
+  __builtin_amdgcn_sched_barrier(0);
+  link_prepare<KP, FED>(p, s, o);
+  link_finish<KP>(p, s, o, red);
 }
 
 // Operands of the hopping force F(x) of f/a_force.f:54-110 for x in parity eo

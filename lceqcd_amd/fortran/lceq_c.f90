@@ -127,6 +127,20 @@ module lceq_c
       type(*) :: f(*)
       integer(c_int64_t), value :: ld
     end function
+    integer(c_int) function lceq_confsave(h, u_path, a_path, real_bytes) &
+        bind(C, name="lceq_confsave")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*) :: u_path, a_path
+      integer(c_int), value :: real_bytes
+    end function
+    integer(c_int) function lceq_oldconf(h, u_path, a_path, real_bytes) &
+        bind(C, name="lceq_oldconf")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*) :: u_path, a_path
+      integer(c_int), value :: real_bytes
+    end function
     integer(c_int) function lceq_unitconf(h) bind(C, name="lceq_unitconf")
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -9,16 +9,21 @@
 ! reference's names (f/measinit.f:246-273) and record formats (f/meas1.f:91,
 ! f/correlator.f:168-181, f/meas2a.f:83-97, f/meas1.f:70-76).
 !
-!   lceq_main [inputfile] [seed] [ntherm]
+!   lceq_main [inputfile] [seed] [ntherm] [realbytes]
 !
-! ntherm (default 100, as f/main.f:20) is the number of thermalisation sweeps.
+! ntherm (default 100, as f/main.f:20) is the number of thermalisation sweeps;
+! realbytes (8, or 4 for files of the as-shipped single-precision build) is
+! the real size of the configuration files.
 !
 ! Differences from the reference, by design: one device holds the whole
 ! lattice (dimproc and nproc are read and ignored); the uniforms come from the
 ! counter-based Philox streams keyed by `seed` (default 314159, the base of the
 ! reference's RANLUX seeds, f/measinit.f:214-217); init = 0 or 1 is the hot
-! start (Haar links, Higgs N(0, 0.5^2)), 2 is unitconf, 4 is testconf; init = 3
-! (reading a saved configuration) is not supported.
+! start (Haar links, Higgs N(0, 0.5^2)), 2 is unitconf, 4 is testconf, 3 is
+! oldconf (f/confget.f:318-526) from the file f/measinit.f:246-250 names, or,
+! when that lower-case name does not exist, from the upper-case name that
+! f/confsave.f:249-253 writes.  The configuration is saved at the end as
+! f/finalize.f:43 does.
 program lceq_main
   use iso_c_binding
   use lceq_c
@@ -33,6 +38,8 @@ program lceq_main
   character(len=49) :: action_file, joblog_file, corr_file, checkp_file, afield_file
   integer :: betaint, xint, yint, isweep, meas, mstart, nmeas, iter1, iter_or, z, ntherm
   integer(c_int64_t) :: seed
+  integer :: realbytes
+  character(len=49) :: u_conf_file, a_conf_file
   type(c_ptr) :: h
   real(8) :: acc1, acc2, acc3, acco, acch, accg, da1, da, obs(7)
   real(8), allocatable :: corr_a(:), corr_u(:), corr_3(:)
@@ -49,6 +56,11 @@ program lceq_main
   if (command_argument_count() >= 3) then
     call get_command_argument(3, arg)
     read(arg, *) ntherm
+  end if
+  realbytes = 8
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg)
+    read(arg, *) realbytes
   end if
 
   ! ---- f/paraminit.f:82-101
@@ -101,6 +113,13 @@ program lceq_main
     call lceq_check(lceq_unitconf(h), 'unitconf')
   case (4)
     call lceq_check(lceq_testconf(h), 'testconf')
+  case (3)
+    call conf_name(u_conf_file, 'u', confstart, .false.)
+    call conf_name(a_conf_file, 'a', confstart, .false.)
+    if (.not. exists(u_conf_file)) call conf_name(u_conf_file, 'u', confstart, .true.)
+    if (.not. exists(a_conf_file)) call conf_name(a_conf_file, 'a', confstart, .true.)
+    call lceq_check(lceq_oldconf(h, trim(u_conf_file) // c_null_char, &
+                                 trim(a_conf_file) // c_null_char, realbytes), 'oldconf')
   case default
     write(*, *) 'init out of range or unsupported:', init
     call abort()
@@ -190,11 +209,37 @@ program lceq_main
   write(u_job, '(a,1x,e10.5)') 'heat-bath acceptance : ', acch / max(nsweep, 1)
   write(u_job, '(a,1x,e10.5)') 'Gauge  acceptance    : ', accg / max(nsweep, 1)
   close(u_job)
+  ! ---- f/finalize.f:43, f/confsave.f:249-253
+  call conf_name(u_conf_file, 'u', confstart + nsweep, .true.)
+  call conf_name(a_conf_file, 'a', confstart + nsweep, .true.)
+  call lceq_check(lceq_confsave(h, trim(u_conf_file) // c_null_char, &
+                                trim(a_conf_file) // c_null_char, realbytes), 'confsave')
   call lceq_check(lceq_destroy(h), 'lceq_destroy')
 
 1000 format(a6, '_s', i2.2, '_be', i5.5, '_x', i5.5, '_y', i5.5, '_u', i6.6)
 
 contains
+
+  ! configuration file names: f/confsave.f:249-253 (saved) and
+  ! f/measinit.f:246-250 (the case oldconf opens)
+  subroutine conf_name(name, field, sweep, saved)
+    character(len=*), intent(out) :: name
+    character(len=1), intent(in) :: field
+    integer, intent(in) :: sweep
+    logical, intent(in) :: saved
+    if (saved) then
+      write(name, 1010) field, nl(3), nint(1000 * beta), nint(10000 * x), nint(10000 * y), sweep
+    else
+      write(name, 1020) field, nl(3), nint(1000 * beta), nint(10000 * x), nint(10000 * y), sweep
+    end if
+1010 format(a1, '_conf_s', i2.2, '_BE', i5.5, '_x', i5.5, '_y', i5.5, '_U', i6.6)
+1020 format(a1, '_conf_s', i2.2, '_be', i5.5, '_x', i5.5, '_y', i5.5, '_u', i6.6)
+  end subroutine
+
+  logical function exists(name)
+    character(len=*), intent(in) :: name
+    inquire(file=trim(name), exist=exists)
+  end function
 
   ! f/measinit.f:64-72 header lines (formats 1001-1003)
   subroutine header(unit, name)

@@ -13,6 +13,7 @@ Parity 0 is the reference's "even" (x+y+z even, 0-based).  There is no CPU
 fallback: every call runs the HIP kernels or raises :class:`LceqError`.
 """
 import ctypes as C
+import os
 import math
 
 import numpy as np
@@ -91,6 +92,17 @@ class Lattice:
 
     def set_imode(self, imode):
         check(self._L.lceq_set_imode(self.h, imode))
+
+    # ------------------------------------------------------ configuration files
+    def confsave(self, u_path, a_path, real_bytes=8):
+        """f/confsave.f: write the lattice in the reference's file format."""
+        check(self._L.lceq_confsave(self.h, None if u_path is None else os.fsencode(u_path),
+                                    None if a_path is None else os.fsencode(a_path), real_bytes))
+
+    def oldconf(self, u_path, a_path, real_bytes=8):
+        """f/confget.f:318 oldconf: load a configuration file into the lattice."""
+        check(self._L.lceq_oldconf(self.h, None if u_path is None else os.fsencode(u_path),
+                                   None if a_path is None else os.fsencode(a_path), real_bytes))
 
     # -------------------------------------------------------------- uniforms
     def rng_philox(self, seed, pass_counter=0):

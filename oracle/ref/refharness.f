@@ -12,6 +12,12 @@
 *     the captured stream.  The extra draw after the run must equal the
 *     captured extra uniform; that proves the consumption count.
 *
+*     Commands 14 (confsave) and 15 (oldconf) run the reference's
+*     configuration I/O in the working directory on a 1x1x1 Cartesian
+*     communicator: 14 writes the confsave files for beta, x = lambda,
+*     y = alpha and sweep number arg1; 15 reads the files u_in and a_in
+*     and writes the fields.
+*
 *     Input  (stream): int32 lx,ly,lz,cmd,arg1,arg2,imode,nrep,seed
 *                      real64 beta,beta_a,beta_2,beta_4,lambda,alpha,
 *                             limit_or,delta
@@ -26,6 +32,8 @@
       INCLUDE 'pointer.inc'
       INCLUDE 'gaugefield.inc'
       INCLUDE 'measurement.inc'
+      INCLUDE 'parallel_mpi_types.inc'
+      INCLUDE 'filenames.inc'
 
       integer(4) hdr(9)
       real(8) par(8)
@@ -38,6 +46,8 @@
       integer, allocatable :: sd(:)
       real acc, delta, da1, rnext
       character(len=512) fin, fout
+      integer dims3(3), ctype, rtype
+      logical per3(3)
 
       call get_command_argument(1, fin)
       call get_command_argument(2, fout)
@@ -250,6 +260,41 @@
         call wr(11, outb, 4*n)
         rnext = nacch
         call wrs(11, rnext)
+      else if (cmd .eq. 14 .or. cmd .eq. 15) then
+        dims3 = 1
+        per3 = .true.
+        call MPI_CART_CREATE(MPI_COMM_WORLD, 3, dims3, per3, .false.,
+     &                       comm, ierror)
+        if (storage_size(u(1,1,0,1)) .eq. 128) then
+          ctype = MPI_DOUBLE_COMPLEX
+          rtype = MPI_DOUBLE_PRECISION
+        else
+          ctype = MPI_COMPLEX
+          rtype = MPI_REAL
+        endif
+        call MPI_TYPE_CONTIGUOUS(nc*ownhalfvol(2), ctype, load_u_type,
+     &                           ierror)
+        call MPI_TYPE_COMMIT(load_u_type, ierror)
+        save_u_type = load_u_type
+        call MPI_TYPE_CONTIGUOUS(3*ownhalfvol(2), rtype, load_a_type,
+     &                           ierror)
+        call MPI_TYPE_COMMIT(load_a_type, ierror)
+        save_a_type = load_a_type
+        u_unit = 20
+        a_unit = 21
+        if (cmd .eq. 14) then
+          x = lambda
+          y = alpha
+          confstart = arg1
+          nsweep = 0
+          call confsave
+        else
+          u_conf_file = 'u_in'
+          a_conf_file = 'a_in'
+          call oldconf
+          call wru(11, n)
+          call wra(11, n)
+        endif
       else if (cmd .eq. 13) then
         allocate(rr(n), xx(n), outb(n))
         do i = 1, n
@@ -385,4 +430,17 @@
         enddo
       enddo
       call wr(iu, ob, 24*n)
+      end
+
+*     Single-rank self messages: confsave and oldconf send each slice to
+*     root with a blocking MPI_SEND before root posts the receive, and
+*     this MPI does not complete a blocking send to self that way.  The
+*     harness therefore defines MPI_SEND as a nonblocking send whose
+*     request is freed; the MPI_RECV that follows in the same routine
+*     completes it before the buffer is touched again.
+      subroutine mpi_send(buf, cnt, dtype, dest, tag, comm0, ierr)
+      implicit none
+      integer buf(*), cnt, dtype, dest, tag, comm0, ierr, req
+      call pmpi_isend(buf, cnt, dtype, dest, tag, comm0, req, ierr)
+      call pmpi_request_free(req, ierr)
       end

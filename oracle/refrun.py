@@ -16,7 +16,7 @@ REF_DIR = os.path.join(_HERE, "_ref")
 
 CMD = dict(geom=1, staples=2, agauge=3, a_force=4, ahiggs1=5, ahiggs2=6,
            ahiggs3=7, renorm=8, or_gauge_fixing=9, landau_gauge=10, sweep=11,
-           heat_kpi=12, gauss=13)
+           heat_kpi=12, gauss=13, confsave=14, oldconf=15)
 
 
 def harness_path(prec=64):
@@ -37,9 +37,11 @@ def build():
 
 def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
         beta=9.0, beta_a=0.0, beta_2=1.0, beta_4=0.0, lam=1.0, alpha=1.0,
-        limit_or=1e-9, delta=0.4, ex1=None, ex2=None, prec=64, exe=None, env=None):
+        limit_or=1e-9, delta=0.4, ex1=None, ex2=None, prec=64, exe=None, env=None,
+        workdir=None):
     """Run one harness command; returns the list of output records.  exe
-    selects another build of the harness (e.g. the drop-in one)."""
+    selects another build of the harness (e.g. the drop-in one); workdir is
+    the working directory (configuration files of confsave / oldconf)."""
     lx, ly, lz = shape
     n = lx * ly * lz // 2
     hdr = np.array([lx, ly, lz, CMD[cmd], arg1, arg2, imode, nrep, seed],
@@ -61,7 +63,7 @@ def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
         e = dict(os.environ)
         e.update(env or {})
         r = subprocess.run([exe or harness_path(prec), fin, fout], env=e, timeout=300,
-                           stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, cwd=workdir)
         if r.returncode != 0:
             raise RuntimeError("harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:],
                                                               r.stderr[-2000:]))

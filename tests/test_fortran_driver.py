@@ -82,3 +82,32 @@ def test_driver_runs_reference_schedule(tmp_path, oracle_lib):
         O.landau_gauge()
         O.correlator()
     np.testing.assert_allclose(rec, np.array(exp), rtol=1e-12)
+    # the configuration saved at the end (f/finalize.f:43) is the final state,
+    # in the reference's file format and upper-case name
+    from lceqcd_amd.conf import conf_read
+    u, a = conf_read((8, 8, 8), tmp_path / "u_conf_s08_BE09000_x01000_y05000_U000002",
+                     tmp_path / "a_conf_s08_BE09000_x01000_y05000_U000002")
+    np.testing.assert_allclose(u, O.get_links(), rtol=0, atol=1e-11)
+    np.testing.assert_allclose(a, O.get_higgs(), rtol=0, atol=1e-11)
+
+
+@pytest.mark.gpu
+def test_driver_restarts_from_saved_configuration(tmp_path):
+    """init = 3 reads the configuration a previous run saved (under the
+    upper-case name, which the lower-case lookup of the reference misses) and
+    a zero-sweep run saves it again unchanged."""
+    exe = os.path.join(FDIR, "lceq_main")
+    from lceqcd_amd.conf import conf_read, conf_write
+    from helpers import rand_conf
+    u, a = rand_conf((8, 8, 8), 9, 1.0)
+    conf_write((8, 8, 8), u, a, tmp_path / "u_conf_s08_BE09000_x01000_y05000_U000000",
+               tmp_path / "a_conf_s08_BE09000_x01000_y05000_U000000")
+    (tmp_path / "inputfile").write_text(INPUTFILE.format(init=3, nsweep="00000"))
+    print("running lceq_main 8^3 init=3 nsweep=0", flush=True)
+    r = subprocess.run([exe, "inputfile", "1", "0"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # nsweep = 0: saved under confstart + nsweep = 0 again, same fields
+    u2, a2 = conf_read((8, 8, 8), tmp_path / "u_conf_s08_BE09000_x01000_y05000_U000000",
+                       tmp_path / "a_conf_s08_BE09000_x01000_y05000_U000000")
+    assert np.array_equal(u2, u) and np.array_equal(a2, a)
