@@ -22,6 +22,16 @@ beta_A in linspace(0.28, 0.40, 12) x beta_2 in linspace(-1.2, -0.6, 8)
 the same launches (one replayed hipGraph per sweep); value = aggregate
 site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
 
+    python bench.py --workload mlp48 | mlp96 [--sweeps K]
+
+the learned Higgs proposal at 48^3 or 96^3 near the transition (trace
+convention beta_G = 9, beta_A = 0.34, beta_2 = -0.9, beta_4 = 0.1): a network
+trained on 2000 plain-sweep configurations, then K timed sweeps of the plain
+sweep (1 KP + 4 OR + ahiggs1) and K of the learned one (1 KP + 4 OR + learned
+Higgs update) from the same configuration; reports acceptance, tau_int of
+Tr A^2 (Madras-Sokal) and effectively independent samples/s = sweeps/s /
+(2 tau_int); value = the learned sweep's independent samples/s.
+
     python bench.py --calibrate-copy
 
 copies a 512 MiB device buffer three times (a plain copy kernel of known
@@ -139,6 +149,92 @@ def replicas96(args):
     print(json.dumps(rec), flush=True)
 
 
+def learned(args, L0):
+    import numpy as np
+    import lceqcd_amd
+    from lceqcd_amd import couplings_from_trace_convention
+    from lceqcd_amd.analysis import tau_int
+    shape = (L0,) * 3
+    V = L0 ** 3
+    coup = couplings_from_trace_convention(9.0, 0.34, -0.9, 0.1)
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(*coup)
+    L.hotstart(12345, 0.5)
+    L.rng_philox(12345, 0)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 300, n_or=4, delta=0.4, measflag=0)
+    # training: 2000 configurations, a subsample of sites from each
+    M = lceqcd_amd.Mlp(seed=1)
+    stride = max(1, V // 2 // 3456)
+    t0 = time.perf_counter()
+    for c in range(args.train_configs):
+        d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 1, n_or=4, delta=d, measflag=0)
+        M.collect(L, nsteps=8, stride=stride)
+    t1 = time.perf_counter()
+    loss = M.train(args.train_steps, batch=8192, lr=2e-3, seed=1)
+    L.sync()
+    t2 = time.perf_counter()
+    u0, a0 = L.get_links(), L.get_higgs()
+
+    def chain(learned_update):
+        X = lceqcd_amd.Lattice(*shape)
+        X.set_couplings(*coup)
+        X.put_links(u0)
+        X.put_higgs(a0)
+        X.rng_philox(777 if learned_update else 778, 0)
+        dd = d
+        hist = []
+        X.sync()
+        ts = time.perf_counter()
+        left = args.sweeps
+        while left > 0:
+            k = min(left, 4000)
+            if learned_update:
+                X.sweep_mlp(M, k, n_or=4, measflag=1)
+            else:
+                dd = X.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, k, n_or=4, delta=dd, measflag=1)
+            hist.append(X.history(k))
+            left -= k
+        X.sync()
+        el = time.perf_counter() - ts
+        h = np.concatenate(hist)
+        tra2 = h[:, 2] / 2.0  # Tr A^2 = |a|^2 / 2 per site
+        tau, dtau, w = tau_int(tra2)
+        sps = args.sweeps / el
+        return {"sweeps_per_s": sps, "site_updates_per_s": V * sps, "tau_int": tau,
+                "tau_int_err": dtau, "window": int(w), "acceptance": float(h[:, 8].mean()),
+                "tr_a2_mean": float(tra2.mean()), "indep_samples_per_s": sps / (2.0 * tau)}
+
+    plain = chain(False)
+    lrn = chain(True)
+    ratio = lrn["indep_samples_per_s"] / plain["indep_samples_per_s"]
+    rec = {
+        "metric": METRIC,
+        "value": lrn["indep_samples_per_s"],
+        "unit": "independent samples/s of <Tr A^2>",
+        "n_gpus": 1,
+        "steps": args.sweeps,
+        "warmup": 300,
+        "ms_per_step": 1e3 / lrn["sweeps_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (network bf16 MFMA, fp32 accumulate)",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d^3 learned Higgs proposal vs plain Metropolis, beta_G=9 beta_A=0.34 "
+                        "beta_2=-0.9 beta_4=0.1 (trace convention), 1 KP + 4 OR per sweep" % L0,
+            "train": {"configs": args.train_configs, "records": M.records(),
+                      "steps": args.train_steps, "batch": 8192,
+                      "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-20:].mean()),
+                      "collect_s": t1 - t0, "train_s": t2 - t1},
+            "plain": plain,
+            "learned": lrn,
+            "learned_over_plain": ratio,
+        },
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def calibrate_copy():
     import torch
     n = 64 * 1024 * 1024  # 64 Mi doubles = 512 MiB per buffer
@@ -163,7 +259,11 @@ def main():
     ap.add_argument("--n-or", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="single64", choices=["single64", "replicas96"])
+    ap.add_argument("--workload", default="single64",
+                    choices=["single64", "replicas96", "mlp48", "mlp96"])
+    ap.add_argument("--sweeps", type=int, default=20000)
+    ap.add_argument("--train-configs", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--hist-out", default=None)
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
@@ -175,6 +275,8 @@ def main():
         return calibrate_copy()
     if args.workload == "replicas96":
         return replicas96(args)
+    if args.workload in ("mlp48", "mlp96"):
+        return learned(args, int(args.workload[3:]))
 
     import lceqcd_amd
     shape = tuple(int(x) for x in args.lattice.split(","))

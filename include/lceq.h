@@ -210,6 +210,43 @@ int lceq_batch_history(lceq_batch *B, double *out, int n);
 /* the current 16 observables of every replica, out[r*16 + i] */
 int lceq_batch_observables(lceq_batch *B, double *out);
 
+/* ---- learned Higgs proposal (no counterpart in the reference) -------------
+ * Per site, a 10 -> 64 -> 64 -> 6 SiLU network (bf16 MFMA inference, fp32
+ * accumulation) maps the local a and the hopping force F of f/a_force.f,
+ * both in the frame whose first axis is F/|F|, |F| and the couplings to the
+ * mean and log-scale of a Gaussian proposal in that frame.  The proposal is
+ * accepted with the FP64 Metropolis-Hastings probability min(1, exp(S(a) -
+ * S(a')) q(a|a')/q(a'|a)), S the site action of f/ahiggs1.f, so the chain is
+ * exact for any weights; the plain Metropolis update (lceq_ahiggs1) is its
+ * fallback.  Training is maximum likelihood on moves of the plain update. */
+#define LCEQ_MLP_NPARAM 5254 /* W1[10][64] b1[64] W2[64][64] b2[64] W3[64][6] b3[6] */
+#define LCEQ_MLP_NSTATE 5274 /* parameters, feature mean[10], feature 1/scale[10] */
+typedef struct lceq_mlp lceq_mlp;
+/* Glorot-uniform weights from seed, zero biases, identity normalisation */
+int lceq_mlp_create(uint64_t seed, lceq_mlp **out);
+int lceq_mlp_destroy(lceq_mlp *M);
+int lceq_mlp_get_state(lceq_mlp *M, float *state, int n);
+int lceq_mlp_set_state(lceq_mlp *M, const float *state, int n);
+/* training records: for each parity of the lattice's current configuration,
+ * the inputs at the current F, nsteps plain Metropolis steps of that parity
+ * (kept in the chain), the new a in the same frame; every stride-th site */
+int lceq_mlp_collect(lceq_lattice *L, lceq_mlp *M, int nsteps, int stride);
+int lceq_mlp_records(lceq_mlp *M, int64_t *n);
+int lceq_mlp_clear(lceq_mlp *M);
+/* steps Adam steps on mini-batches of `batch` records (a multiple of 32);
+ * fit_norm first sets the input normalisation from the records; loss (or
+ * NULL) receives the mean negative log-likelihood of every step */
+int lceq_mlp_train(lceq_mlp *M, int steps, int batch, double lr, uint64_t seed, int fit_norm,
+                   double *loss);
+/* the inference network on n rows of the 10 raw inputs: out[n][6] */
+int lceq_mlp_eval(lceq_mlp *M, const float *x, int n, float *out);
+/* one Higgs sweep (both parities) with the learned proposal; accept is the
+ * acceptance; the Higgs observables (rho, rho^2, a.F) are always measured */
+int lceq_higgs_mlp(lceq_lattice *L, lceq_mlp *M, double *accept);
+/* nsweep sweeps of 1 KP + n_or overrelaxation + the learned Higgs update,
+ * history rows as lceq_sweep with measflag */
+int lceq_sweep_mlp(lceq_lattice *L, lceq_mlp *M, int nsweep, int n_or, int measflag);
+
 /* ---- configuration files (host) -------------------------------------------
  * The reference's direct-access unformatted configuration files
  * (f/confsave.f:50-253, f/confget.f:318-526): link records per (mu, parity,

@@ -69,6 +69,17 @@ SIGNATURES = {
     "lceq_batch_history": [VP, DP, I],
     "lceq_batch_observables": [VP, DP],
     "lceq_dims": [VP, IP],
+    "lceq_mlp_create": [U64, C.POINTER(VP)],
+    "lceq_mlp_destroy": [VP],
+    "lceq_mlp_get_state": [VP, C.POINTER(C.c_float), I],
+    "lceq_mlp_set_state": [VP, C.POINTER(C.c_float), I],
+    "lceq_mlp_collect": [VP, VP, I, I],
+    "lceq_mlp_records": [VP, C.POINTER(I64)],
+    "lceq_mlp_clear": [VP],
+    "lceq_mlp_train": [VP, I, I, D, U64, I, DP],
+    "lceq_mlp_eval": [VP, C.POINTER(C.c_float), I, C.POINTER(C.c_float)],
+    "lceq_higgs_mlp": [VP, VP, DP],
+    "lceq_sweep_mlp": [VP, VP, I, I, I],
     "lceq_conf_write": [IP, DP, DP, C.c_char_p, C.c_char_p, I],
     "lceq_conf_read": [IP, DP, DP, C.c_char_p, C.c_char_p, I],
     "lceq_confsave": [VP, C.c_char_p, C.c_char_p, I],

@@ -38,7 +38,7 @@ int lceq::set_error(int code, const char* fmt, ...) {
 #define NEEDL(L) \
   if (!(L)) return fail(LCEQ_ERR_ARG, "null lattice handle")
 
-static int check_launch() {
+int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(LCEQ_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
   return LCEQ_OK;
@@ -232,7 +232,7 @@ static int take_uniforms(lceq_lattice* L, int64_t count, const double** ptr) {
 }
 
 // next pass number for philox mode (fed mode leaves the counter alone)
-static uint64_t next_pass(lceq_lattice* L) {
+uint64_t next_pass(lceq_lattice* L) {
   uint64_t p = L->pass;
   if (L->rng_mode == LCEQ_RNG_PHILOX) L->pass += 1;
   return p;
@@ -341,14 +341,14 @@ extern "C" int lceq_hotstart(lceq_lattice* L, uint64_t seed, double sigma) {
   return check_launch();
 }
 
-static int read_obs(lceq_lattice* L, double* host) {
+int read_obs(lceq_lattice* L, double* host) {
   HIPCHK(hipMemcpyAsync(host, L->obs, LCEQ_OBS_N * sizeof(double), hipMemcpyDeviceToHost,
                         L->stream));
   HIPCHK(hipStreamSynchronize(L->stream));
   return LCEQ_OK;
 }
 
-static int write_obs(lceq_lattice* L, int index, double value) {
+int write_obs(lceq_lattice* L, int index, double value) {
   HIPCHK(hipMemcpyAsync(L->obs + index, &value, sizeof(double), hipMemcpyHostToDevice, L->stream));
   HIPCHK(hipStreamSynchronize(L->stream));
   return LCEQ_OK;
@@ -396,7 +396,7 @@ extern "C" int lceq_heat_kpi(lceq_lattice* L, int n, const double* b, double* v,
 // one full link sweep (eo outer, mu inner as in f/agauge.f:53-57); with
 // reduce the six launches leave their plaquette and acceptance partial sums
 // in slots 0-5 for launch_finalize
-static int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce) {
+int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce) {
   int64_t per = (kp ? 7 : 1) * (int64_t)L->g.n;
   const double* fed0;
   int rc = take_uniforms(L, 6 * per, &fed0);

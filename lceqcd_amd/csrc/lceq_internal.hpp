@@ -124,3 +124,12 @@ void launch_unitconf(lceq_lattice* L);
 void launch_testconf(lceq_lattice* L);
 
 }  // namespace lceq
+
+// helpers of the C ABI layer (lceq_api.hip) shared with the other entry files
+int check_launch();
+uint64_t next_pass(lceq_lattice* L);
+int read_obs(lceq_lattice* L, double* host);
+int write_obs(lceq_lattice* L, int index, double value);
+// one full link sweep (six (eo, mu) launches) of the KP heatbath or the
+// overrelaxation, partials in slots 0-5 when reduce is set
+int link_sweep(lceq_lattice* L, int kp, int measflag, int renorm_on_write, int reduce);

@@ -93,6 +93,17 @@ class Lattice:
     def set_imode(self, imode):
         check(self._L.lceq_set_imode(self.h, imode))
 
+    # ------------------------------------------------------ learned proposal
+    def higgs_mlp(self, mlp):
+        """One Higgs sweep with the learned proposal (exact MH); returns accept."""
+        acc = C.c_double()
+        check(self._L.lceq_higgs_mlp(self.h, mlp.h, C.byref(acc)))
+        return acc.value
+
+    def sweep_mlp(self, mlp, nsweep, n_or=4, measflag=1):
+        """1 KP + n_or OR + the learned Higgs update per sweep."""
+        check(self._L.lceq_sweep_mlp(self.h, mlp.h, int(nsweep), int(n_or), int(measflag)))
+
     # ------------------------------------------------------ configuration files
     def confsave(self, u_path, a_path, real_bytes=8):
         """f/confsave.f: write the lattice in the reference's file format."""

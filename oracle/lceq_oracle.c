@@ -1084,6 +1084,166 @@ void oc_polyakov(oc_lattice *L, double *mean_sq, double *corr) {
   free(sl);
 }
 
+/* ---- learned Higgs proposal (liblceq's lceq_higgs_mlp; no reference
+ * counterpart).  Action difference as f/ahiggs1.f:64-92. */
+#define MLP_NIN 10
+#define MLP_NH 64
+#define MLP_NOUT 6
+#define MLP_OFF_B1 (MLP_NIN * MLP_NH)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_NH)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_NH * MLP_NH)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_NH)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_NH * MLP_NOUT)
+#define MLP_NPARAM (MLP_OFF_B3 + MLP_NOUT)
+
+static float bf16_round(float f) { /* round to nearest even, as the GPU conversion */
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  u &= 0xFFFF0000u;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
+static void mlp_layer(const float *x, int nin, const float *w, const float *b, int nout, float *y,
+                      int act) {
+  for (int n = 0; n < nout; ++n) {
+    double acc = 0.0;
+    for (int k = 0; k < nin; ++k) acc += (double)x[k] * (double)bf16_round(w[k * nout + n]);
+    float z = (float)acc + b[n];
+    y[n] = act ? bf16_round(z / (1.0f + expf(-z))) : z;
+  }
+}
+
+static void mlp_forward(const float *st, const float *raw, float *out) {
+  float x[MLP_NIN], h1[MLP_NH], h2[MLP_NH];
+  for (int k = 0; k < MLP_NIN; ++k)
+    x[k] = bf16_round((raw[k] - st[MLP_NPARAM + k]) * st[MLP_NPARAM + MLP_NIN + k]);
+  mlp_layer(x, MLP_NIN, st, st + MLP_OFF_B1, MLP_NH, h1, 1);
+  mlp_layer(h1, MLP_NH, st + MLP_OFF_W2, st + MLP_OFF_B2, MLP_NH, h2, 1);
+  mlp_layer(h2, MLP_NH, st + MLP_OFF_W3, st + MLP_OFF_B3, MLP_NOUT, out, 0);
+}
+
+/* frame with first axis F/|F|; second axis from z (or x when F is within
+ * acos(0.9) of z) */
+static void mlp_frame(const double F[3], double e[3][3], double *fn) {
+  *fn = sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+  double e1[3] = {1.0, 0.0, 0.0};
+  if (*fn > 0.0)
+    for (int i = 0; i < 3; ++i) e1[i] = F[i] / *fn;
+  double h[3] = {0.0, 0.0, 1.0};
+  if (fabs(e1[2]) >= 0.9) {
+    h[0] = 1.0;
+    h[2] = 0.0;
+  }
+  double e2[3] = {h[1] * e1[2] - h[2] * e1[1], h[2] * e1[0] - h[0] * e1[2],
+                  h[0] * e1[1] - h[1] * e1[0]};
+  double n2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+  for (int i = 0; i < 3; ++i) e2[i] = e2[i] / n2;
+  double e3[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                  e1[0] * e2[1] - e1[1] * e2[0]};
+  for (int i = 0; i < 3; ++i) {
+    e[0][i] = e1[i];
+    e[1][i] = e2[i];
+    e[2][i] = e3[i];
+  }
+}
+
+static void mlp_raw(const double y[3], double fn, const oc_lattice *L, float *x) {
+  x[0] = (float)y[0];
+  x[1] = (float)y[1];
+  x[2] = (float)y[2];
+  x[3] = (float)fn;
+  x[4] = 0.0f;
+  x[5] = 0.0f;
+  x[6] = (float)fn;
+  x[7] = (float)L->beta_a;
+  x[8] = (float)L->beta_2;
+  x[9] = (float)L->beta_4;
+}
+
+static void mlp_proposal(const float *o, double mu[3], double sig[3], double ls[3]) {
+  for (int i = 0; i < 3; ++i) {
+    mu[i] = (double)o[i];
+    ls[i] = (double)fminf(fmaxf(o[3 + i], -7.0f), 4.0f);
+    sig[i] = exp(ls[i]);
+  }
+}
+
+static double mlp_logq(const double y[3], const double mu[3], const double sig[3],
+                       const double ls[3]) {
+  double r = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double t = (y[i] - mu[i]) / sig[i];
+    r = r - ls[i] - 0.5 * (t * t);
+  }
+  return r;
+}
+
+void oc_higgs_mlp(oc_lattice *L, const float *st, double *accept) {
+  const double TPI = 2.0 * acos(-1.0);
+  int n = L->n;
+  double acc = 0.0;
+  higgs_reset(L);
+  for (int eo = 0; eo < 2; ++eo) {
+    uint64_t pass = L->pass;
+    L->pass += 1;
+    oc_a_force(L, eo);
+    double s2 = 0.0, s4 = 0.0, sk = 0.0;
+    for (int s = 0; s < n; ++s) {
+      double *a = APTR(L, L->a, eo, s);
+      const double *F = L->force + 3 * (size_t)s;
+      double e[3][3], fn, ya[3], yp[3], ypr[3], an[3];
+      mlp_frame(F, e, &fn);
+      for (int i = 0; i < 3; ++i) ya[i] = (e[i][0] * a[0] + e[i][1] * a[1]) + e[i][2] * a[2];
+      float raw[MLP_NIN], o1[MLP_NOUT], o2[MLP_NOUT];
+      mlp_raw(ya, fn, L, raw);
+      mlp_forward(st, raw, o1);
+      double mu[3], sig[3], ls[3], mu2[3], sig2[3], ls2[3];
+      mlp_proposal(o1, mu, sig, ls);
+      double u0 = oc_pu(L, pass, s, 0), u1 = oc_pu(L, pass, s, 1);
+      double u2 = oc_pu(L, pass, s, 2), u3 = oc_pu(L, pass, s, 3), racc = oc_pu(L, pass, s, 4);
+      double r01 = sqrt(-2.0 * log(u0)), r23 = sqrt(-2.0 * log(u2));
+      double z[3] = {r01 * cos(TPI * u1), r01 * sin(TPI * u1), r23 * cos(TPI * u3)};
+      for (int i = 0; i < 3; ++i) yp[i] = mu[i] + sig[i] * z[i];
+      for (int i = 0; i < 3; ++i) an[i] = (e[0][i] * yp[0] + e[1][i] * yp[1]) + e[2][i] * yp[2];
+      for (int i = 0; i < 3; ++i) ypr[i] = (e[i][0] * an[0] + e[i][1] * an[1]) + e[i][2] * an[2];
+      mlp_raw(ypr, fn, L, raw);
+      mlp_forward(st, raw, o2);
+      mlp_proposal(o2, mu2, sig2, ls2);
+      double lqf = mlp_logq(ypr, mu, sig, ls), lqr = mlp_logq(ya, mu2, sig2, ls2);
+      double sfo = 0.0, sfn = 0.0, ro = 0.0, rn = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        sfo = sfo + a[i] * F[i];
+        sfn = sfn + an[i] * F[i];
+        ro = ro + a[i] * a[i];
+        rn = rn + an[i] * an[i];
+      }
+      double boltz = exp(-(L->beta_a * (sfo - sfn)) + L->beta_2 * (ro - rn) +
+                         L->beta_4 * (ro * ro - rn * rn) + (lqr - lqf));
+      if (boltz >= racc) {
+        a[0] = an[0];
+        a[1] = an[1];
+        a[2] = an[2];
+        ro = rn;
+        acc += 1.0;
+      }
+      s2 += ro;
+      s4 += ro * ro;
+      sk += (a[0] * F[0] + a[1] * F[1]) + a[2] * F[2];
+    }
+    L->a02 += s2;
+    L->a04 += s4;
+    L->a0kin += sk;
+  }
+  double vol = (double)L->v;
+  L->a02 /= vol;
+  L->a04 /= vol;
+  L->a0kin /= (6.0 * vol);
+  *accept = acc / vol;
+}
+
 void oc_observables(const oc_lattice *L, double *out) {
   out[0] = L->plaq0;
   out[1] = L->a0kin;

@@ -90,6 +90,15 @@ void oc_polyakov(oc_lattice *L, double *mean_sq, double *corr);
  * out[0..6] = plaq0, a0kin, a02, a04, a0f(1..3) */
 void oc_observables(const oc_lattice *L, double *out);
 
+/* The learned Higgs proposal of liblceq (no reference counterpart): both
+ * parities, network state = 5254 parameters + feature mean[10] + 1/scale[10]
+ * (the library's layout).  The network is evaluated as the GPU's bf16
+ * inference does (bf16 inputs, weights and hidden activations, products
+ * summed in double and rounded to float, fp32 bias and SiLU); the frame, the
+ * Box-Muller draws and the FP64 Metropolis-Hastings test are those of the
+ * library.  Philox mode only.  Measures like ahiggs2 with measflag = 1. */
+void oc_higgs_mlp(oc_lattice *L, const float *state, double *accept);
+
 /* Philox4x32-10 block (Salmon et al., SC'11), for known-answer tests */
 void oc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                       uint32_t out[4]);

@@ -70,6 +70,7 @@ def lib():
             "oc_ahiggs1": (None, [VP, DP, I, DP]),
             "oc_ahiggs2": (None, [VP, I, DP]),
             "oc_ahiggs3": (None, [VP, I, DP]),
+            "oc_higgs_mlp": (None, [VP, C.POINTER(C.c_float), DP]),
             "oc_renorm": (None, [VP]),
             "oc_u_f_renorm": (None, [VP]),
             "oc_or_gauge_fixing": (None, [VP, DP]),
@@ -255,6 +256,13 @@ class OracleLattice:
     def ahiggs3(self, measflag):
         acc = D(0)
         self._L.oc_ahiggs3(self.h, measflag, C.byref(acc))
+        return acc.value
+
+    def higgs_mlp(self, state):
+        """The learned Higgs proposal with network state (5274 float32)."""
+        st = np.ascontiguousarray(state, dtype=np.float32)
+        acc = D(0)
+        self._L.oc_higgs_mlp(self.h, st.ctypes.data_as(C.POINTER(C.c_float)), C.byref(acc))
         return acc.value
 
     def renorm(self):
