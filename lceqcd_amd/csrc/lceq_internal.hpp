@@ -71,7 +71,7 @@ enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC
 
 int blocks_for(int n);
 // sites per thread of the link launches, and their block count for n sites
-constexpr int LINK_SPT = 2;
+constexpr int LINK_SPT = 1;
 int link_blocks(int n);
 
 // Event brackets around runs of launches of the profiled class.  Brackets

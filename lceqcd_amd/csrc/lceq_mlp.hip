@@ -114,10 +114,14 @@ __host__ __device__ inline void raw_features(const double y[3], double fn, doubl
 
 // ----------------------------------------------------------------- inference
 
+// Per-wave scratch of one evaluation: bf16 inputs (row = lane), reused for
+// the fp32 outputs, and the bf16 hidden activations
 struct alignas(16) MlpLds {
-  unsigned short x[64 * 32];  // bf16 inputs, row = lane
-  unsigned short h[64 * 64];  // bf16 hidden activations
-  float o[64 * 16];           // outputs
+  union {
+    unsigned short x[64 * 32];
+    float o[64 * 16];
+  };
+  unsigned short h[64 * 64];
 };
 
 __device__ __forceinline__ unsigned short to_bf16_bits(float v) {
@@ -127,17 +131,55 @@ __device__ __forceinline__ unsigned short to_bf16_bits(float v) {
   return u;
 }
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+// inference SiLU: hardware exp and reciprocal (the activations are rounded
+// to bf16 next; exactness of the chain only needs a deterministic network)
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+// Order this wave's LDS accesses: every lane of a wave writes and reads only
+// its own wave's scratch, and one wave's LDS operations complete in program
+// order, so the evaluation needs no block barrier.
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// This lane's share of the inference weights, loaded once per kernel so
+// that their latency hides under the site loads
+struct MlpRegs {
+  bf16x8 w1[4], w2[4][2], w3[2];
+  float b1[4], b2[4], b3;
+};
+
+__device__ __forceinline__ void mlp_load(const MlpPack* __restrict__ P, MlpRegs& R) {
+  const int l = threadIdx.x & 63, col = l & 15;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    R.w1[nt] = P->w1[nt][l];
+    R.w2[nt][0] = P->w2[nt][0][l];
+    R.w2[nt][1] = P->w2[nt][1][l];
+    R.b1[nt] = P->b1[16 * nt + col];
+    R.b2[nt] = P->b2[16 * nt + col];
+  }
+  R.w3[0] = P->w3[0][l];
+  R.w3[1] = P->w3[1][l];
+  R.b3 = P->b3[col];
+}
 
 // One evaluation for the 64 sites of this wave (row = lane).  Every lane of
-// the block must call it (it synchronises the block).
-__device__ void mlp_eval(const MlpPack* __restrict__ P, const float (&xin)[NIN], float (&out)[NOUT],
-                         MlpLds& S) {
+// the wave must call it.
+__device__ void mlp_eval(const MlpPack* __restrict__ P, const MlpRegs& R, const float (&xin)[NIN],
+                         float (&out)[NOUT], MlpLds& S) {
   const int l = threadIdx.x & 63, col = l & 15, grp = l >> 4;
-  // inputs, normalised, as a bf16 row padded to 32
+  // inputs, normalised, as a bf16 row padded to 32: four 16-byte stores
+  unsigned short row[32];
 #pragma unroll
   for (int k = 0; k < 32; ++k)
-    S.x[l * 32 + k] = k < NIN ? to_bf16_bits((xin[k] - P->xmean[k]) * P->xinv[k]) : (unsigned short)0;
-  __syncthreads();
+    row[k] = k < NIN ? to_bf16_bits((xin[k] - P->xmean[k]) * P->xinv[k]) : (unsigned short)0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bf16x8 v;
+    memcpy(&v, &row[8 * q], 16);
+    *reinterpret_cast<bf16x8*>(&S.x[l * 32 + 8 * q]) = v;
+  }
+  wave_sync();
   // layer 1: [64 x 32] x [32 x 64]
   floatx4 acc[4][4];
 #pragma unroll
@@ -145,8 +187,7 @@ __device__ void mlp_eval(const MlpPack* __restrict__ P, const float (&xin)[NIN],
     bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.x[(16 * mt + col) * 32 + 8 * grp]);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, P->w1[nt][l], floatx4{0, 0, 0, 0},
-                                                            0, 0, 0);
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, R.w1[nt], floatx4{0, 0, 0, 0}, 0, 0, 0);
   }
   // D of a 16 x 16 tile: lane l holds column l % 16, rows 4 (l / 16) .. + 3
 #pragma unroll
@@ -156,9 +197,9 @@ __device__ void mlp_eval(const MlpPack* __restrict__ P, const float (&xin)[NIN],
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         int m = 16 * mt + 4 * grp + i, n = 16 * nt + col;
-        S.h[m * 64 + n] = to_bf16_bits(silu(acc[mt][nt][i] + P->b1[n]));
+        S.h[m * 64 + n] = to_bf16_bits(silu_fast(acc[mt][nt][i] + R.b1[nt]));
       }
-  __syncthreads();
+  wave_sync();
   // layer 2: [64 x 64] x [64 x 64]
   bf16x8 a2[4][2];
 #pragma unroll
@@ -166,14 +207,14 @@ __device__ void mlp_eval(const MlpPack* __restrict__ P, const float (&xin)[NIN],
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
       a2[mt][kb] = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 32 * kb + 8 * grp]);
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][0], P->w2[nt][0][l],
+      floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][0], R.w2[nt][0],
                                                           floatx4{0, 0, 0, 0}, 0, 0, 0);
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][1], P->w2[nt][1][l], c, 0, 0, 0);
+      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][1], R.w2[nt][1], c, 0, 0, 0);
     }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
@@ -182,23 +223,23 @@ __device__ void mlp_eval(const MlpPack* __restrict__ P, const float (&xin)[NIN],
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         int m = 16 * mt + 4 * grp + i, n = 16 * nt + col;
-        S.h[m * 64 + n] = to_bf16_bits(silu(acc[mt][nt][i] + P->b2[n]));
+        S.h[m * 64 + n] = to_bf16_bits(silu_fast(acc[mt][nt][i] + R.b2[nt]));
       }
-  __syncthreads();
+  wave_sync();
   // layer 3: [64 x 64] x [64 x 16]
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     bf16x8 x0 = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 8 * grp]);
     bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 32 + 8 * grp]);
-    floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0, P->w3[0][l], floatx4{0, 0, 0, 0}, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1, P->w3[1][l], c, 0, 0, 0);
+    floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0, R.w3[0], floatx4{0, 0, 0, 0}, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1, R.w3[1], c, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) S.o[(16 * mt + 4 * grp + i) * 16 + col] = c[i] + P->b3[col];
+    for (int i = 0; i < 4; ++i) S.o[(16 * mt + 4 * grp + i) * 16 + col] = c[i] + R.b3;
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int j = 0; j < NOUT; ++j) out[j] = S.o[l * 16 + j];
-  __syncthreads();
+  wave_sync();
 }
 
 // Gaussian proposal of the network outputs: mean and scale in the frame
@@ -248,6 +289,8 @@ __global__ __launch_bounds__(256) void k_higgs_mlp(MlpArgs p) {
   bool valid = s < (unsigned)p.g.n;
   unsigned sl = valid ? s : 0u;
   const Geom& g = p.g;
+  MlpRegs W;
+  mlp_load(p.pack, W);
   Coord c = site_coord(g, p.eo, (int)sl);
   ForceLinks k;
   load_force_links(p.U, p.A, g, p.eo, sl, c, k);
@@ -263,7 +306,7 @@ __global__ __launch_bounds__(256) void k_higgs_mlp(MlpArgs p) {
   to_frame(fr, av.x, ya);
   float xin[NIN], o1[NOUT], o2[NOUT];
   raw_features(ya, fr.fn, p.beta_a, p.beta_2, p.beta_4, xin);
-  mlp_eval(p.pack, xin, o1, S);
+  mlp_eval(p.pack, W, xin, o1, S);
   double mu[3], sig[3], ls[3];
   proposal(o1, mu, sig, ls);
   // Box-Muller
@@ -275,7 +318,7 @@ __global__ __launch_bounds__(256) void k_higgs_mlp(MlpArgs p) {
   from_frame(fr, yp, an);
   to_frame(fr, an, ypr);
   raw_features(ypr, fr.fn, p.beta_a, p.beta_2, p.beta_4, xin);
-  mlp_eval(p.pack, xin, o2, S);
+  mlp_eval(p.pack, W, xin, o2, S);
   double mu2[3], sig2[3], ls2[3];
   proposal(o2, mu2, sig2, ls2);
   double lqf = log_q(ypr, mu, sig, ls), lqr = log_q(ya, mu2, sig2, ls2);
@@ -312,9 +355,11 @@ __global__ __launch_bounds__(256) void k_mlp_eval(const MlpPack* __restrict__ P,
   int i = blockIdx.x * TB + threadIdx.x;
   int il = i < n ? i : n - 1;
   float xin[NIN], o[NOUT];
+  MlpRegs W;
+  mlp_load(P, W);
 #pragma unroll
   for (int k = 0; k < NIN; ++k) xin[k] = x[(size_t)il * NIN + k];
-  mlp_eval(P, xin, o, lds[threadIdx.x >> 6]);
+  mlp_eval(P, W, xin, o, lds[threadIdx.x >> 6]);
   if (i < n)
 #pragma unroll
     for (int j = 0; j < NOUT; ++j) y[(size_t)i * NOUT + j] = o[j];
