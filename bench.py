@@ -11,6 +11,9 @@ step), with plaquette / <rho> / <rho^2> / <a.F> accumulated every sweep.
 
     python bench.py [--steps K] [--warmup W] [--lattice 64,64,64] [--cpu-seconds S]
 
+Before the timed region the autotuner picks the launch shape (threads per
+block, XCD-aware block order) from 10 sweeps per shape and restores the chain
+(--shape T,X fixes it instead, e.g. for profiling).
 Prints one JSON line.  value = site-updates/s (V sites per sweep).
 
     python bench.py --workload replicas96 [--steps K] [--hist-out FILE]
@@ -19,7 +22,8 @@ measures the batched-replica scan instead: 96 independent lattices cycling
 through 16^3, 24^3, 32^3, 40^3, 48^3 and 24^2x96, on the grid
 beta_A in linspace(0.28, 0.40, 12) x beta_2 in linspace(-1.2, -0.6, 8)
 (trace convention, beta_G = 9, beta_4 = 0.1), seeds 12345 + r, all swept by
-the same launches (one replayed hipGraph per sweep); value = aggregate
+the same launches (replayed hipGraphs of 8 sweeps, launch shape picked by
+the autotuner, which leaves the chains unchanged); value = aggregate
 site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
 
     python bench.py --workload mlp48 | mlp96 [--sweeps K]
@@ -110,6 +114,7 @@ def replicas96(args):
     B.hotstart(seeds, 0.5)
     B.rng_philox(seeds)
     B.sweep(args.warmup, n_or=args.n_or, delta=0.4, measflag=0)
+    tuned = B.autotune(n_or=args.n_or, trials=3)
     B.sync()
     t0 = time.perf_counter()
     B.sweep(args.steps, n_or=args.n_or, measflag=1)
@@ -134,12 +139,16 @@ def replicas96(args):
         "data": "synthetic",
         "config": {
             "workload": "96 replicas (16^3..48^3, 24^2x96) on a beta_A x beta_2 grid, 1 KP + %d OR "
-                        "+ ahiggs1 per sweep, one hipGraph per sweep" % args.n_or,
+                        "+ ahiggs1 per sweep, hipGraph of 8 sweeps, autotuned launch shape"
+                        % args.n_or,
             "total_sites": V,
             "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
             "sweep_algorithmic_frac_of_hbm": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9
             / HBM_PEAK_GBPS,
             "tr_a2_mean_range": [float(tra2.mean(0).min()), float(tra2.mean(0).max())],
+            "launch_shape": {"threads_per_block": B.get_shape()[0], "xcd_order": B.get_shape()[1]},
+            "autotune_ms_per_sweep": {"%d%s" % (t, "x" if x else ""): v
+                                      for (t, x), v in tuned.items()},
         },
     }
     if args.hist_out:
@@ -270,6 +279,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--hist-out", default=None)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="threads_per_block,xcd_order for the link and Higgs launches "
+                         "(skips the autotuner; e.g. 256,1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -291,6 +303,12 @@ def main():
     L.rng_philox(12345, 0)
     d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.warmup, n_or=args.n_or, delta=0.4, measflag=1)
     L.history(args.warmup)
+    if args.shape:
+        t, x = (int(v) for v in args.shape.split(","))
+        L.set_shape(t, bool(x))
+        tuned = {}
+    else:
+        tuned = L.autotune(n_or=args.n_or, trials=10)
     L.sync()
 
     # timed region: K sweeps, synchronised on both sides
@@ -336,6 +354,9 @@ def main():
             "lattice": list(shape),
             "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
             "link_updates_per_s": 3 * V * (1 + args.n_or) / (ms * 1e-3),
+            "launch_shape": {"threads_per_block": L.get_shape()[0], "xcd_order": L.get_shape()[1]},
+            "autotune_ms_per_sweep": {"%d%s" % (t, "x" if x else ""): v
+                                      for (t, x), v in tuned.items()},
         },
         "roofline": {
             "bound": "hbm",

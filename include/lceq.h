@@ -75,6 +75,18 @@ int lceq_set_gauge_fixing(lceq_lattice *L, double lambda, double alpha, double l
                           int max_iter);
 /* imode of f/ahiggs2.f:52-77 (0 plain, 1 alpha-corrected width) */
 int lceq_set_imode(lceq_lattice *L, int imode);
+/* launch shape of the link and Higgs updates (no counterpart in the
+ * reference): threads (= sites) per block 64, 128, 256 or 512, XCD-aware
+ * block order (1) or launch order (0).  Fields do not depend on the shape;
+ * plaquette and Higgs sums are added in a shape-dependent order.  Default
+ * 256, XCD order. */
+int lceq_set_shape(lceq_lattice *L, int threads_per_block, int xcd_order);
+int lceq_get_shape(lceq_lattice *L, int *threads_per_block, int *xcd_order);
+/* autotuner: time `trials` production sweeps (lceq_sweep schedule 0 with
+ * n_or) at every shape and keep the fastest; fields, observables, the pass
+ * counter and the history are restored.  ms (or NULL) receives 8 times per
+ * sweep, shapes (64,128,256,512) x (launch order, XCD order).  Philox only. */
+int lceq_autotune(lceq_lattice *L, int n_or, int trials, double *ms);
 
 /* ---- uniforms --------------------------------------------------------- */
 /* Counter-based Philox4x32-10 streams: draw k of site s in update pass P uses
@@ -201,7 +213,8 @@ int lceq_batch_get_higgs(lceq_batch *B, int r, double *a, int64_t ld);
 /* nsweep sweeps of 1 KP + n_or OR + ahiggs1 for every replica; delta (nrep
  * values, or NULL to keep the device values, initially 0.4) seeds the
  * ahiggs1 steps; measflag records one history row per replica per sweep;
- * use_graph replays one captured hipGraph per sweep.  Queued, no sync. */
+ * use_graph replays a hipGraph of 8 captured sweeps (the remainder of nsweep
+ * is launched directly).  Queued, no sync. */
 int lceq_batch_sweep(lceq_batch *B, int nsweep, int n_or, const double *delta, int measflag,
                      int use_graph);
 int lceq_batch_sync(lceq_batch *B);
@@ -209,6 +222,18 @@ int lceq_batch_sync(lceq_batch *B);
 int lceq_batch_history(lceq_batch *B, double *out, int n);
 /* the current 16 observables of every replica, out[r*16 + i] */
 int lceq_batch_observables(lceq_batch *B, double *out);
+/* launch shape of the batch's update kernels (no counterpart in the
+ * reference): threads (= sites) per block 64, 128, 256 or 512, and the
+ * XCD-aware block order (1) or the launch order (0).  Fields do not depend on
+ * the shape; the plaquette and Higgs sums are added in a shape-dependent
+ * order.  Default 256, XCD order. */
+int lceq_batch_set_shape(lceq_batch *B, int threads_per_block, int xcd_order);
+int lceq_batch_get_shape(lceq_batch *B, int *threads_per_block, int *xcd_order);
+/* autotuner: time `trials` sweeps of every shape on this batch, keep the
+ * fastest; fields, observables and the sweep counter are restored.  ms (or
+ * NULL) receives 8 times per sweep, shapes (64,128,256,512) x (launch order,
+ * XCD order). */
+int lceq_batch_autotune(lceq_batch *B, int n_or, int trials, double *ms);
 
 /* ---- learned Higgs proposal (no counterpart in the reference) -------------
  * Per site, a 10 -> 64 -> 64 -> 6 SiLU network (bf16 MFMA inference, fp32

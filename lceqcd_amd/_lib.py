@@ -16,6 +16,9 @@ SIGNATURES = {
     "lceq_set_couplings": [VP, D, D, D, D],
     "lceq_set_gauge_fixing": [VP, D, D, D, I],
     "lceq_set_imode": [VP, I],
+    "lceq_set_shape": [VP, I, I],
+    "lceq_get_shape": [VP, IP, IP],
+    "lceq_autotune": [VP, I, I, DP],
     "lceq_rng_philox": [VP, U64, U64],
     "lceq_rng_feed": [VP, DP, I64],
     "lceq_rng_state": [VP, C.POINTER(I64), C.POINTER(U64)],
@@ -68,6 +71,9 @@ SIGNATURES = {
     "lceq_batch_sync": [VP],
     "lceq_batch_history": [VP, DP, I],
     "lceq_batch_observables": [VP, DP],
+    "lceq_batch_set_shape": [VP, I, I],
+    "lceq_batch_get_shape": [VP, IP, IP],
+    "lceq_batch_autotune": [VP, I, I, DP],
     "lceq_dims": [VP, IP],
     "lceq_mlp_create": [U64, C.POINTER(VP)],
     "lceq_mlp_destroy": [VP],

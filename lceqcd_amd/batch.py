@@ -97,3 +97,20 @@ class Batch:
         out = np.zeros((self.nrep, 16))
         check(self._L.lceq_batch_observables(self.h, _dp(out)))
         return out
+
+    def set_shape(self, threads_per_block=256, xcd_order=True):
+        """Launch shape of the update kernels (same fields for every shape)."""
+        check(self._L.lceq_batch_set_shape(self.h, int(threads_per_block), 1 if xcd_order else 0))
+
+    def get_shape(self):
+        t, x = C.c_int(), C.c_int()
+        check(self._L.lceq_batch_get_shape(self.h, C.byref(t), C.byref(x)))
+        return t.value, bool(x.value)
+
+    def autotune(self, n_or=4, trials=3):
+        """Time every launch shape on this batch and keep the fastest (the
+        chains are left where they were).  Returns {(threads, xcd): ms/sweep}."""
+        ms = np.zeros(8)
+        check(self._L.lceq_batch_autotune(self.h, int(n_or), int(trials), _dp(ms)))
+        return {((64, 128, 256, 512)[i // 2], bool(i % 2)): float(ms[i]) for i in range(8)}
+

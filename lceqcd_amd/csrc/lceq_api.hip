@@ -66,7 +66,9 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   L->vol = lx * ly * lz;
   g.n = L->vol / 2;
   g.npad = (g.n + 63) & ~63;
-  L->pstride = blocks_for(g.n) + 8;
+  L->tb = 256;
+  L->xcd = 1;
+  L->pstride = (g.n + 63) / 64 + 8;  // block partials of the smallest launch shape
   L->beta = 9.0;
   L->lambda = 1.0;
   L->alpha = 1.0;
@@ -172,6 +174,81 @@ extern "C" int lceq_dims(lceq_lattice* L, int* dims) {
   dims[1] = L->g.ly;
   dims[2] = L->g.lz;
   return LCEQ_OK;
+}
+
+extern "C" int lceq_set_shape(lceq_lattice* L, int threads_per_block, int xcd_order) {
+  NEEDL(L);
+  int t = threads_per_block;
+  if (t != 64 && t != 128 && t != 256 && t != 512)
+    return fail(LCEQ_ERR_ARG, "threads per block must be 64, 128, 256 or 512");
+  HIPCHK(hipStreamSynchronize(L->stream));
+  L->tb = t;
+  L->xcd = xcd_order ? 1 : 0;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_get_shape(lceq_lattice* L, int* threads_per_block, int* xcd_order) {
+  NEEDL(L);
+  if (threads_per_block) *threads_per_block = L->tb;
+  if (xcd_order) *xcd_order = L->xcd;
+  return LCEQ_OK;
+}
+
+// Time `trials` production sweeps (schedule 0, no measurement) at every
+// launch shape and keep the fastest; the fields, observables, pass counter
+// and history are restored, so tuning leaves the chain where it was.
+extern "C" int lceq_autotune(lceq_lattice* L, int n_or, int trials, double* ms) {
+  NEEDL(L);
+  if (trials < 1 || n_or < 0) return fail(LCEQ_ERR_ARG, "bad autotune arguments");
+  if (L->rng_mode != LCEQ_RNG_PHILOX) return fail(LCEQ_ERR_ARG, "autotune needs Philox streams");
+  size_t np = (size_t)L->g.npad;
+  std::vector<double> u(24 * np), a(6 * np), o(LCEQ_OBS_N);
+  HIPCHK(hipMemcpyAsync(u.data(), L->U, u.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipMemcpyAsync(a.data(), L->A, a.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipMemcpyAsync(o.data(), L->obs, o.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  uint64_t pass0 = L->pass;
+  int rows0 = L->hist_rows;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(LCEQ_ERR_HIP, "hipEventCreate failed");
+  }
+  const int shapes[4] = {64, 128, 256, 512};
+  double best = 1e300, delta = o[OBS_DELTA];
+  int best_tb = L->tb, best_x = L->xcd, rc = LCEQ_OK;
+  for (int i = 0; i < 8 && rc == LCEQ_OK; ++i) {
+    L->tb = shapes[i / 2];
+    L->xcd = i % 2;
+    if ((rc = lceq_sweep(L, LCEQ_SWEEP_KP_OR_METRO, 1, n_or, &delta, 0))) break;
+    (void)hipEventRecord(e0, L->stream);
+    if ((rc = lceq_sweep(L, LCEQ_SWEEP_KP_OR_METRO, trials, n_or, &delta, 0))) break;
+    (void)hipEventRecord(e1, L->stream);
+    float el = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+      rc = fail(LCEQ_ERR_HIP, "timing failed");
+      break;
+    }
+    double per = (double)el / trials;
+    if (ms) ms[i] = per;
+    if (per < best) {
+      best = per;
+      best_tb = L->tb;
+      best_x = L->xcd;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  L->tb = best_tb;
+  L->xcd = best_x;
+  L->pass = pass0;
+  L->hist_rows = rows0;
+  HIPCHK(hipMemcpyAsync(L->U, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipMemcpyAsync(L->A, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipMemcpyAsync(L->obs, o.data(), o.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  return rc;
 }
 
 extern "C" int lceq_set_imode(lceq_lattice* L, int imode) {

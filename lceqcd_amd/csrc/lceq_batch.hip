@@ -1,9 +1,11 @@
 // lceq_batch.hip -- many independent lattices of uneven sizes and couplings
 // (a phase-diagram scan) swept together: one launch per (parity, mu) link
 // class and per Higgs parity covers every replica, so the device sees the sum
-// of all replicas' sites per launch.  A sweep's launches are captured once in
-// a hipGraph and replayed; the Philox pass number comes from a device-side
-// sweep counter so that the captured graph stays valid from sweep to sweep.
+// of all replicas' sites per launch.  Eight sweeps' launches are captured
+// once in a hipGraph and replayed; the Philox pass number comes from a
+// device-side sweep counter so that the captured graph stays valid from sweep
+// to sweep.  The launch shape (threads per block, XCD-aware block order) is a
+// runtime choice that lceq_batch_autotune picks by timing.
 //
 // Replica r with seed s_r follows exactly the trajectory that a single
 // lceq_lattice with seed s_r, started at pass 0, follows under
@@ -24,7 +26,7 @@ using namespace lceq;
 
 namespace {
 
-constexpr int TB = 256;
+constexpr int TB_MIN = 64;  // smallest block of the launch shapes
 constexpr int FIN_TB = 1024;
 
 struct Replica {
@@ -44,22 +46,24 @@ struct BatchCommon {
   int total_nb;
   const uint64_t* sweep;  // device sweep counter
   int passes_per_sweep;
+  int xcd;                // XCD-aware block order (else launch order)
 };
 
 __device__ __forceinline__ int batch_block(const BatchCommon& c, int& r, const Replica*& R) {
-  int gblk = xcd_block(c.total_nb);
+  int gblk = c.xcd ? xcd_block(c.total_nb) : (int)blockIdx.x;
   r = c.blk2rep[gblk];
   R = c.reps + r;
   return gblk - R->blk0;
 }
 
-template <bool KP>
-__global__ __launch_bounds__(256) void k_link_batch(BatchCommon c, int eo, int mu, int pass_in_sweep,
+// One (eo, mu) link class of every replica, TBX sites per block
+template <bool KP, int TBX>
+__global__ __launch_bounds__(TBX) void k_link_batch(BatchCommon c, int eo, int mu, int pass_in_sweep,
                                                     int measflag, int reduce) {
   int r;
   const Replica* R;
   int lblk = batch_block(c, r, R);
-  unsigned s = (unsigned)(lblk * TB + threadIdx.x);
+  unsigned s = (unsigned)(lblk * TBX + threadIdx.x);
   double red[3] = {0.0, 0.0, 0.0};
   if (s < (unsigned)R->g.n) {
     LinkSite p;
@@ -78,16 +82,17 @@ __global__ __launch_bounds__(256) void k_link_batch(BatchCommon c, int eo, int m
     link_update_site<KP, false>(p, s, red);
   }
   if (reduce)
-    block_sum_store<3>(red, R->part + (size_t)(eo * 3 + mu) * LCEQ_NQ * R->pstride, R->pstride,
-                       lblk);
+    block_sum_store<3, TBX / 64>(red, R->part + (size_t)(eo * 3 + mu) * LCEQ_NQ * R->pstride,
+                                 R->pstride, lblk);
 }
 
-__global__ __launch_bounds__(256) void k_higgs_batch(BatchCommon c, int eo, int pass_in_sweep,
+template <int TBX>
+__global__ __launch_bounds__(TBX) void k_higgs_batch(BatchCommon c, int eo, int pass_in_sweep,
                                                      int measflag) {
   int r;
   const Replica* R;
   int lblk = batch_block(c, r, R);
-  unsigned s = (unsigned)(lblk * TB + threadIdx.x);
+  unsigned s = (unsigned)(lblk * TBX + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
   if (s < (unsigned)R->g.n) {
     HiggsSite p;
@@ -106,7 +111,8 @@ __global__ __launch_bounds__(256) void k_higgs_batch(BatchCommon c, int eo, int 
     p.measflag = measflag;
     higgs_update_site<1, false>(p, s, red);
   }
-  block_sum_store<4>(red, R->part + (size_t)(6 + eo) * LCEQ_NQ * R->pstride, R->pstride, lblk);
+  block_sum_store<4, TBX / 64>(red, R->part + (size_t)(6 + eo) * LCEQ_NQ * R->pstride, R->pstride,
+                               lblk);
 }
 
 // Per-replica close of a sweep (block r): the same reduction and slot order
@@ -199,6 +205,9 @@ __global__ void k_tick(uint64_t* sweep) {
       return bfail(LCEQ_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
   } while (0)
 
+struct lceq_batch;
+static int batch_upload_table(lceq_batch* B);
+
 struct lceq_batch {
   int nrep;
   std::vector<Replica> reps;  // host copy of the device table
@@ -213,9 +222,12 @@ struct lceq_batch {
   uint64_t* d_sweep;
   uint64_t sweeps_done;  // host mirror of the device counter
   hipStream_t stream;
-  hipGraphExec_t graph;
+  hipGraphExec_t graph;  // GRAPH_SWEEPS sweeps
   int graph_n_or, graph_meas;
+  int tb, xcd;           // launch shape
 };
+
+constexpr int GRAPH_SWEEPS = 8;
 
 static int batch_upload_table(lceq_batch* B) {
   BHIP(hipMemcpyAsync(B->d_reps, B->reps.data(), B->reps.size() * sizeof(Replica),
@@ -223,6 +235,32 @@ static int batch_upload_table(lceq_batch* B) {
   BHIP(hipStreamSynchronize(B->stream));
   return LCEQ_OK;
 }
+
+// Block tables for launch shape tb: replica r owns blocks [blk0, blk0 + nb)
+// of every update launch.  Partial-sum slots are sized for the smallest
+// shape at creation (pstride), so a shape change needs no reallocation.
+static int batch_layout(lceq_batch* B, int tb, int xcd) {
+  std::vector<int> blk2rep;
+  int nb_total = 0;
+  for (int r = 0; r < B->nrep; ++r) {
+    Replica& R = B->reps[r];
+    R.nb = (R.g.n + tb - 1) / tb;
+    R.blk0 = nb_total;
+    for (int k = 0; k < R.nb; ++k) blk2rep.push_back(r);
+    nb_total += R.nb;
+  }
+  BHIP(hipMemcpyAsync(B->d_blk2rep, blk2rep.data(), blk2rep.size() * sizeof(int),
+                      hipMemcpyHostToDevice, B->stream));
+  B->total_nb = nb_total;
+  B->tb = tb;
+  B->xcd = xcd;
+  if (B->graph) {
+    BHIP(hipGraphExecDestroy(B->graph));
+    B->graph = nullptr;
+  }
+  return batch_upload_table(B);
+}
+
 
 extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   if (!out || !dims || nrep < 1) return bfail(LCEQ_ERR_ARG, "bad batch arguments");
@@ -249,7 +287,7 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
     R.g.h2 = lx * ly / 2;
     R.g.n = lx * ly * lz / 2;
     R.g.npad = (R.g.n + 63) & ~63;
-    R.nb = (R.g.n + TB - 1) / TB;
+    R.nb = (R.g.n + TB_MIN - 1) / TB_MIN;
     R.blk0 = nb_total;
     R.pstride = R.nb;
     R.beta = 9.0;
@@ -299,9 +337,7 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   (void)hipMemcpyAsync(B->obs, obs0.data(), obs0.size() * sizeof(double), hipMemcpyHostToDevice,
                        B->stream);
   (void)hipMemsetAsync(B->d_sweep, 0, sizeof(uint64_t), B->stream);
-  if (hipMemcpyAsync(B->d_blk2rep, blk2rep.data(), blk2rep.size() * sizeof(int),
-                     hipMemcpyHostToDevice, B->stream) != hipSuccess ||
-      batch_upload_table(B) != LCEQ_OK) {
+  if (batch_layout(B, 256, 1) != LCEQ_OK) {
     lceq_batch_destroy(B);
     return bfail(LCEQ_ERR_HIP, "batch initialisation failed");
   }
@@ -406,11 +442,9 @@ extern "C" int lceq_batch_get_higgs(lceq_batch* B, int r, double* a, int64_t ld)
   return batch_field(B, r, a, nullptr, ld, 0);
 }
 
-// Enqueue one sweep of every replica: 1 KP + n_or overrelaxation link
-// sweeps + ahiggs1, then the per-replica finalize and the counter tick.
-static void enqueue_sweep(lceq_batch* B, int n_or, int measflag) {
-  BatchCommon c{B->d_reps, B->d_blk2rep, B->total_nb, B->d_sweep, 6 * (1 + n_or) + 2};
-  dim3 grid(B->total_nb), blk(TB);
+template <int TBX>
+static void enqueue_updates(lceq_batch* B, const BatchCommon& c, int n_or, int measflag) {
+  dim3 grid(B->total_nb), blk(TBX);
   int pass = 0;
   for (int k = -1; k < n_or; ++k) {
     int kp = k < 0;
@@ -418,15 +452,29 @@ static void enqueue_sweep(lceq_batch* B, int n_or, int measflag) {
     for (int eo = 0; eo < 2; ++eo)
       for (int mu = 0; mu < 3; ++mu) {
         if (kp)
-          hipLaunchKernelGGL(k_link_batch<true>, grid, blk, 0, B->stream, c, eo, mu, pass, m, m);
+          hipLaunchKernelGGL((k_link_batch<true, TBX>), grid, blk, 0, B->stream, c, eo, mu, pass,
+                             m, m);
         else
-          hipLaunchKernelGGL(k_link_batch<false>, grid, blk, 0, B->stream, c, eo, mu, pass, m, m);
+          hipLaunchKernelGGL((k_link_batch<false, TBX>), grid, blk, 0, B->stream, c, eo, mu, pass,
+                             m, m);
         ++pass;
       }
   }
   for (int eo = 0; eo < 2; ++eo) {
-    hipLaunchKernelGGL(k_higgs_batch, grid, blk, 0, B->stream, c, eo, pass, measflag);
+    hipLaunchKernelGGL((k_higgs_batch<TBX>), grid, blk, 0, B->stream, c, eo, pass, measflag);
     ++pass;
+  }
+}
+
+// Enqueue one sweep of every replica: 1 KP + n_or overrelaxation link
+// sweeps + ahiggs1, then the per-replica finalize and the counter tick.
+static void enqueue_sweep(lceq_batch* B, int n_or, int measflag) {
+  BatchCommon c{B->d_reps, B->d_blk2rep, B->total_nb, B->d_sweep, 6 * (1 + n_or) + 2, B->xcd};
+  switch (B->tb) {
+    case 64: enqueue_updates<64>(B, c, n_or, measflag); break;
+    case 128: enqueue_updates<128>(B, c, n_or, measflag); break;
+    case 512: enqueue_updates<512>(B, c, n_or, measflag); break;
+    default: enqueue_updates<256>(B, c, n_or, measflag); break;
   }
   int link_kind = measflag ? (n_or ? 2 : 1) : 0;
   hipLaunchKernelGGL(k_finalize_batch, dim3(B->nrep), dim3(FIN_TB), 0, B->stream, B->d_reps,
@@ -443,10 +491,8 @@ extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const doubl
                           hipMemcpyHostToDevice, B->stream));
     BHIP(hipStreamSynchronize(B->stream));
   }
-  if (!use_graph) {
-    for (int it = 0; it < nsweep; ++it) enqueue_sweep(B, n_or, measflag);
-    BHIP(hipGetLastError());
-  } else {
+  int direct = nsweep;
+  if (use_graph && nsweep >= GRAPH_SWEEPS) {
     if (!B->graph || B->graph_n_or != n_or || B->graph_meas != measflag) {
       if (B->graph) {
         BHIP(hipGraphExecDestroy(B->graph));
@@ -454,7 +500,7 @@ extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const doubl
       }
       hipGraph_t g;
       BHIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_sweep(B, n_or, measflag);
+      for (int k = 0; k < GRAPH_SWEEPS; ++k) enqueue_sweep(B, n_or, measflag);
       BHIP(hipStreamEndCapture(B->stream, &g));
       hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -465,8 +511,11 @@ extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const doubl
       B->graph_n_or = n_or;
       B->graph_meas = measflag;
     }
-    for (int it = 0; it < nsweep; ++it) BHIP(hipGraphLaunch(B->graph, B->stream));
+    for (int it = 0; it < nsweep / GRAPH_SWEEPS; ++it) BHIP(hipGraphLaunch(B->graph, B->stream));
+    direct = nsweep % GRAPH_SWEEPS;
   }
+  for (int it = 0; it < direct; ++it) enqueue_sweep(B, n_or, measflag);
+  BHIP(hipGetLastError());
   B->sweeps_done += nsweep;
   return LCEQ_OK;
 }
@@ -500,3 +549,78 @@ extern "C" int lceq_batch_observables(lceq_batch* B, double* out) {
                  hipMemcpyDeviceToHost));
   return LCEQ_OK;
 }
+
+extern "C" int lceq_batch_set_shape(lceq_batch* B, int threads_per_block, int xcd_order) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  int t = threads_per_block;
+  if (t != 64 && t != 128 && t != 256 && t != 512)
+    return bfail(LCEQ_ERR_ARG, "threads per block must be 64, 128, 256 or 512");
+  BHIP(hipStreamSynchronize(B->stream));
+  return batch_layout(B, t, xcd_order ? 1 : 0);
+}
+
+extern "C" int lceq_batch_get_shape(lceq_batch* B, int* threads_per_block, int* xcd_order) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  if (threads_per_block) *threads_per_block = B->tb;
+  if (xcd_order) *xcd_order = B->xcd;
+  return LCEQ_OK;
+}
+
+// Time every launch shape on `trials` sweeps of this batch (after one
+// untimed sweep per shape) and keep the fastest.  The fields, observables
+// and sweep counter are restored afterwards, so tuning leaves the chains
+// where they were.  ms (optional, 8 values) receives the time per sweep of
+// shapes (64, 128, 256, 512) x (launch order, XCD order).
+extern "C" int lceq_batch_autotune(lceq_batch* B, int n_or, int trials, double* ms) {
+  if (!B || trials < 1 || n_or < 0) return bfail(LCEQ_ERR_ARG, "bad autotune arguments");
+  BHIP(hipStreamSynchronize(B->stream));
+  double *pool0 = nullptr, *obs0 = nullptr;
+  uint64_t* sw0 = nullptr;
+  size_t pbytes = B->pool_doubles * sizeof(double), obytes = (size_t)B->nrep * LCEQ_OBS_N * sizeof(double);
+  BHIP(hipMalloc((void**)&pool0, pbytes));
+  int rc = LCEQ_OK;
+  if (hipMalloc((void**)&obs0, obytes) != hipSuccess || hipMalloc((void**)&sw0, 8) != hipSuccess)
+    rc = bfail(LCEQ_ERR_HIP, "hipMalloc failed");
+  if (!rc && (hipMemcpyAsync(pool0, B->pool, pbytes, hipMemcpyDeviceToDevice, B->stream) ||
+              hipMemcpyAsync(obs0, B->obs, obytes, hipMemcpyDeviceToDevice, B->stream) ||
+              hipMemcpyAsync(sw0, B->d_sweep, 8, hipMemcpyDeviceToDevice, B->stream)))
+    rc = bfail(LCEQ_ERR_HIP, "snapshot failed");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) || hipEventCreate(&e1))) rc = bfail(LCEQ_ERR_HIP, "events");
+  const int shapes[4] = {64, 128, 256, 512};
+  double best = 1e300;
+  int best_tb = 256, best_x = 1;
+  for (int i = 0; i < 8 && !rc; ++i) {
+    int tb = shapes[i / 2], x = i % 2;
+    if ((rc = batch_layout(B, tb, x))) break;
+    enqueue_sweep(B, n_or, 0);
+    (void)hipEventRecord(e0, B->stream);
+    for (int t = 0; t < trials; ++t) enqueue_sweep(B, n_or, 0);
+    (void)hipEventRecord(e1, B->stream);
+    float el = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+      rc = bfail(LCEQ_ERR_HIP, "timing failed");
+      break;
+    }
+    double per = (double)el / trials;
+    if (ms) ms[i] = per;
+    if (per < best) {
+      best = per;
+      best_tb = tb;
+      best_x = x;
+    }
+  }
+  if (!rc && (hipMemcpyAsync(B->pool, pool0, pbytes, hipMemcpyDeviceToDevice, B->stream) ||
+              hipMemcpyAsync(B->obs, obs0, obytes, hipMemcpyDeviceToDevice, B->stream) ||
+              hipMemcpyAsync(B->d_sweep, sw0, 8, hipMemcpyDeviceToDevice, B->stream) ||
+              hipStreamSynchronize(B->stream)))
+    rc = bfail(LCEQ_ERR_HIP, "restore failed");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(pool0);
+  if (obs0) (void)hipFree(obs0);
+  if (sw0) (void)hipFree(sw0);
+  if (rc) return rc;
+  return batch_layout(B, best_tb, best_x);
+}
+

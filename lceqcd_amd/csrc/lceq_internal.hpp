@@ -54,6 +54,9 @@ struct lceq_lattice {
   int max_iter, imode;
   // RNG
   int rng_mode;  // 0 fed, 1 philox
+  // launch shape of the link and Higgs updates: threads (= sites) per block
+  // 64/128/256/512 and the XCD-aware block order (partials sized for 64)
+  int tb, xcd;
   uint64_t seed, pass;
   // timing of one kernel class (bench instrumentation): one event pair per
   // run of consecutive launches of that class, launches counted inside runs
@@ -70,9 +73,8 @@ int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3
 enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
 
 int blocks_for(int n);
-// sites per thread of the link launches, and their block count for n sites
-constexpr int LINK_SPT = 1;
-int link_blocks(int n);
+// blocks of the link and Higgs update launches at the lattice's launch shape
+int shape_blocks(const lceq_lattice* L);
 
 // Event brackets around runs of launches of the profiled class.  Brackets
 // nest (a sweep brackets its launches, each launcher brackets itself); only
@@ -102,8 +104,10 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
 // One-block reduction of the partial sums into the observables: link_kind
 // 0 none, 1 KP (agauge), 2 overrelaxation; higgs_kind 0 none, 1-3 ahiggs1-3;
 // with hist_row the observables are also copied into that history row.
+// higgs_nb: block partials of the Higgs slots when their launch used another
+// shape than the lattice's (0: the lattice's shape)
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
-                     int higgs_meas, double* hist_row);
+                     int higgs_meas, double* hist_row, int higgs_nb = 0);
 void launch_staples(lceq_lattice* L, int eo, int mu);
 void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,

@@ -34,35 +34,37 @@ struct LinkArgs {
   LinkSite site;
   int reduce;
   double* part;  // partial sums of this launch's slot: [LCEQ_NQ][pstride]
-  int pstride, nb;
+  int pstride, nb, xcd;
 };
 
-// LINK_SPT sites per thread, TB * LINK_SPT consecutive sites per block.  A
-// thread issues the loads of all its sites before the first update, so the
-// arithmetic of its first site overlaps the transfer of the later ones.
-// Sites past the end load site 0 and skip the update.
-template <bool KP, bool FED>
-__global__ __launch_bounds__(256) void k_link(LinkArgs a) {
-  int blk = xcd_block(a.nb);
-  unsigned n = (unsigned)a.site.g.n;
-  unsigned s[LINK_SPT];
-  LinkOperands o[LINK_SPT];
-#pragma unroll
-  for (int j = 0; j < LINK_SPT; ++j) {
-    s[j] = (unsigned)((blk * LINK_SPT + j) * TB + threadIdx.x);
-    link_load<KP, FED>(a.site, s[j] < n ? s[j] : 0u, o[j]);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int j = 0; j < LINK_SPT; ++j) link_prepare<KP, FED>(a.site, s[j] < n ? s[j] : 0u, o[j]);
-  double red[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int j = 0; j < LINK_SPT; ++j)
-    if (s[j] < n) link_finish<KP>(a.site, s[j], o[j], red);
-  if (a.reduce) block_sum_store<3>(red, a.part, a.pstride, blk);
+// Block index in the launch's block order: XCD-aware (contiguous sites per
+// XCD) or the launch order
+__device__ __forceinline__ int shape_block(int nb, int xcd) {
+  return xcd ? xcd_block(nb) : (int)blockIdx.x;
 }
 
-int link_blocks(int n) { return (n + TB * LINK_SPT - 1) / (TB * LINK_SPT); }
+// One site per thread, TBX sites per block
+template <bool KP, bool FED, int TBX>
+__global__ __launch_bounds__(TBX) void k_link(LinkArgs a) {
+  int blk = shape_block(a.nb, a.xcd);
+  unsigned s = (unsigned)(blk * TBX + threadIdx.x);
+  double red[3] = {0.0, 0.0, 0.0};
+  if (s < (unsigned)a.site.g.n) link_update_site<KP, FED>(a.site, s, red);
+  if (a.reduce) block_sum_store<3, TBX / 64>(red, a.part, a.pstride, blk);
+}
+
+int shape_blocks(const lceq_lattice* L) { return (L->g.n + L->tb - 1) / L->tb; }
+
+template <bool KP, bool FED>
+static void link_dispatch(lceq_lattice* L, const LinkArgs& a) {
+  dim3 grid(a.nb);
+  switch (L->tb) {
+    case 64: hipLaunchKernelGGL((k_link<KP, FED, 64>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_link<KP, FED, 128>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_link<KP, FED, 512>), grid, dim3(512), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((k_link<KP, FED, 256>), grid, dim3(256), 0, L->stream, a); break;
+  }
+}
 
 void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int renorm_on_write,
                  const double* fed, uint64_t pass, int reduce) {
@@ -82,20 +84,21 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.reduce = reduce;
   a.part = L->partials + (size_t)(eo * 3 + mu) * LCEQ_NQ * L->pstride;
   a.pstride = L->pstride;
-  a.nb = link_blocks(L->g.n);
+  a.nb = shape_blocks(L);
+  a.xcd = L->xcd;
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
   prof_count(L, kc);
   if (kp) {
     if (fed)
-      hipLaunchKernelGGL((k_link<true, true>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+      link_dispatch<true, true>(L, a);
     else
-      hipLaunchKernelGGL((k_link<true, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+      link_dispatch<true, false>(L, a);
   } else {
     if (fed)
-      hipLaunchKernelGGL((k_link<false, true>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+      link_dispatch<false, true>(L, a);
     else
-      hipLaunchKernelGGL((k_link<false, false>), dim3(a.nb), dim3(TB), 0, L->stream, a);
+      link_dispatch<false, false>(L, a);
   }
   prof_end(L, kc);
 }
@@ -192,12 +195,12 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
 }
 
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,
-                     int higgs_meas, double* hist_row) {
+                     int higgs_meas, double* hist_row, int higgs_nb) {
   FinalizeArgs a;
   a.part = L->partials;
   a.pstride = L->pstride;
-  a.nb_link = link_blocks(L->g.n);
-  a.nb_higgs = blocks_for(L->g.n);
+  a.nb_link = shape_blocks(L);
+  a.nb_higgs = higgs_nb > 0 ? higgs_nb : shape_blocks(L);
   a.link_kind = link_kind;
   a.link_meas = link_meas;
   a.higgs_kind = higgs_kind;
@@ -240,16 +243,27 @@ void launch_force(lceq_lattice* L, int eo) {
 struct HiggsArgs {
   HiggsSite site;
   double* part;
-  int pstride, nb;
+  int pstride, nb, xcd;
 };
 
-template <int KIND, bool FED>
-__global__ __launch_bounds__(256) void k_higgs(HiggsArgs a) {
-  int blk = xcd_block(a.nb);
-  unsigned s = (unsigned)(blk * TB + threadIdx.x);
+template <int KIND, bool FED, int TBX>
+__global__ __launch_bounds__(TBX) void k_higgs(HiggsArgs a) {
+  int blk = shape_block(a.nb, a.xcd);
+  unsigned s = (unsigned)(blk * TBX + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
   if (s < (unsigned)a.site.g.n) higgs_update_site<KIND, FED>(a.site, s, red);
-  block_sum_store<4>(red, a.part, a.pstride, blk);
+  block_sum_store<4, TBX / 64>(red, a.part, a.pstride, blk);
+}
+
+template <int KIND, bool FED>
+static void higgs_dispatch(lceq_lattice* L, const HiggsArgs& a) {
+  dim3 grid(a.nb);
+  switch (L->tb) {
+    case 64: hipLaunchKernelGGL((k_higgs<KIND, FED, 64>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_higgs<KIND, FED, 128>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_higgs<KIND, FED, 512>), grid, dim3(512), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((k_higgs<KIND, FED, 256>), grid, dim3(256), 0, L->stream, a); break;
+  }
 }
 
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
@@ -270,23 +284,17 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.site.measflag = measflag;
   a.part = L->partials + (size_t)(6 + eo) * LCEQ_NQ * L->pstride;
   a.pstride = L->pstride;
-  a.nb = blocks_for(L->g.n);
-  dim3 grid(a.nb), blk(TB);
+  a.nb = shape_blocks(L);
+  a.xcd = L->xcd;
   prof_begin(L, KC_HIGGS);
   prof_count(L, KC_HIGGS);
-#define LCEQ_HIGGS_LAUNCH(K)                                                         \
-  if (fed)                                                                           \
-    hipLaunchKernelGGL((k_higgs<K, true>), grid, blk, 0, L->stream, a);              \
-  else                                                                               \
-    hipLaunchKernelGGL((k_higgs<K, false>), grid, blk, 0, L->stream, a);
   if (kind == 1) {
-    LCEQ_HIGGS_LAUNCH(1)
+    if (fed) higgs_dispatch<1, true>(L, a); else higgs_dispatch<1, false>(L, a);
   } else if (kind == 2) {
-    LCEQ_HIGGS_LAUNCH(2)
+    if (fed) higgs_dispatch<2, true>(L, a); else higgs_dispatch<2, false>(L, a);
   } else {
-    LCEQ_HIGGS_LAUNCH(3)
+    if (fed) higgs_dispatch<3, true>(L, a); else higgs_dispatch<3, false>(L, a);
   }
-#undef LCEQ_HIGGS_LAUNCH
   prof_end(L, KC_HIGGS);
 }
 

@@ -871,7 +871,7 @@ extern "C" int lceq_higgs_mlp(lceq_lattice* L, lceq_mlp* M, double* accept) {
   if (!L || !M) return mfail(LCEQ_ERR_ARG, "null argument");
   int rc = mlp_higgs_sweep(L, M);
   if (rc) return rc;
-  launch_finalize(L, 0, 0, 2, 1, nullptr);
+  launch_finalize(L, 0, 0, 2, 1, nullptr, blocks_for(L->g.n));
   if (accept) {
     double o[LCEQ_OBS_N];
     if ((rc = read_obs(L, o))) return rc;
@@ -897,7 +897,7 @@ extern "C" int lceq_sweep_mlp(lceq_lattice* L, lceq_mlp* M, int nsweep, int n_or
       if ((rc = link_sweep(L, 0, m, 1, m))) return rc;
     }
     if ((rc = mlp_higgs_sweep(L, M))) return rc;
-    launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 2, 1, row);
+    launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 2, 1, row, blocks_for(L->g.n));
   }
   return check_launch();
 }

@@ -115,6 +115,22 @@ class Lattice:
         check(self._L.lceq_oldconf(self.h, None if u_path is None else os.fsencode(u_path),
                                    None if a_path is None else os.fsencode(a_path), real_bytes))
 
+    def set_shape(self, threads_per_block=256, xcd_order=True):
+        """Launch shape of the link and Higgs updates (same fields)."""
+        check(self._L.lceq_set_shape(self.h, int(threads_per_block), 1 if xcd_order else 0))
+
+    def get_shape(self):
+        t, x = C.c_int(), C.c_int()
+        check(self._L.lceq_get_shape(self.h, C.byref(t), C.byref(x)))
+        return t.value, bool(x.value)
+
+    def autotune(self, n_or=4, trials=5):
+        """Time every launch shape on production sweeps, keep the fastest;
+        the chain is restored.  Returns {(threads, xcd): ms per sweep}."""
+        ms = np.zeros(8)
+        check(self._L.lceq_autotune(self.h, int(n_or), int(trials), _dp(ms)))
+        return {((64, 128, 256, 512)[i // 2], bool(i % 2)): float(ms[i]) for i in range(8)}
+
     # -------------------------------------------------------------- uniforms
     def rng_philox(self, seed, pass_counter=0):
         check(self._L.lceq_rng_philox(self.h, seed, pass_counter))

@@ -1,0 +1,45 @@
+"""Launch shapes of the single-lattice link and Higgs updates (threads per
+block, XCD-aware block order): the fields are the same bit for bit for every
+shape, and the autotuner leaves the chain where it was."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+COUP = (9.0, 0.175, -0.45, 0.025)
+
+
+def _run(shape_, lat=(10, 6, 12), tune=False):
+    import lceqcd_amd
+    L = lceqcd_amd.Lattice(*lat)
+    L.set_couplings(*COUP)
+    L.hotstart(4, 0.5)
+    L.rng_philox(4, 0)
+    d = L.sweep(0, 2, n_or=2, delta=0.4, measflag=0)
+    if shape_:
+        L.set_shape(*shape_)
+    ms = L.autotune(n_or=2, trials=2) if tune else None
+    d = L.sweep(0, 3, n_or=2, delta=d, measflag=1)
+    out = L.get_links(), L.get_higgs(), L.history(3), d, L.get_shape(), ms
+    L.close()
+    return out
+
+
+@pytest.mark.parametrize("shape", [(64, False), (64, True), (128, True), (512, False)])
+def test_shapes_give_the_same_fields(shape):
+    print("single-lattice shape", shape, flush=True)
+    u0, a0, h0, d0, _, _ = _run(None)
+    u1, a1, h1, d1, got, _ = _run(shape)
+    assert got == shape
+    assert np.array_equal(u0, u1) and np.array_equal(a0, a1)
+    assert d0 == d1
+    np.testing.assert_allclose(h1, h0, rtol=1e-13, atol=1e-15)
+
+
+def test_autotune_restores_the_chain():
+    u0, a0, h0, d0, _, _ = _run(None)
+    u1, a1, h1, d1, got, ms = _run(None, tune=True)
+    print("autotune", {k: round(v, 4) for k, v in ms.items()}, "->", got, flush=True)
+    assert ms[got] == min(ms.values())
+    assert np.array_equal(u0, u1) and np.array_equal(a0, a1)
+    np.testing.assert_allclose(h1, h0, rtol=1e-13, atol=1e-15)
