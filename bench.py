@@ -114,7 +114,12 @@ def replicas96(args):
     B.hotstart(seeds, 0.5)
     B.rng_philox(seeds)
     B.sweep(args.warmup, n_or=args.n_or, delta=0.4, measflag=0)
-    tuned = B.autotune(n_or=args.n_or, trials=3)
+    if args.shape:
+        t, x = (int(v) for v in args.shape.split(","))
+        B.set_shape(t, bool(x))
+        tuned = {}
+    else:
+        tuned = B.autotune(n_or=args.n_or, trials=3)
     B.sync()
     t0 = time.perf_counter()
     B.sweep(args.steps, n_or=args.n_or, measflag=1)
