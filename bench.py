@@ -36,6 +36,15 @@ Higgs update) from the same configuration; reports acceptance, tau_int of
 Tr A^2 (Madras-Sokal) and effectively independent samples/s = sweeps/s /
 (2 tau_int); value = the learned sweep's independent samples/s.
 
+    python bench.py --workload refsched64 [--steps K]
+
+the reference's own schedule (f/main.f:37-41) at 64^3 with the couplings of
+its shipped inputfile (beta = 9, x = 0.1, y = 0.5, imode 0): one step =
+ahiggs2 + ahiggs3 + agauge + renorm; value = site-updates/s of that sweep.
+Then Landau gauge fixing (lambda = 1.7, alpha = 1, limit_or = 1e-10) from the
+last configuration, timed per iteration, and the three z-correlator
+measurements; each dominant launch is timed with HIP events.
+
     python bench.py --calibrate-copy
 
 copies a 512 MiB device buffer three times (a plain copy kernel of known
@@ -167,6 +176,111 @@ def replicas96(args):
     print(json.dumps(rec), flush=True)
 
 
+REF_SWEEP_BYTES_PER_SITE = 2 * 2 * HIGGS_PASS_BYTES_PER_SITE + 6 * LINK_PASS_BYTES_PER_SITE + 192
+GFIX_PASS_BYTES_PER_SITE = 216   # all links read + written, a_f of one parity read + written
+SLICES_BYTES_PER_SITE = 144      # u_f 96 + a_f 24 + a 24
+
+
+def refsched64(args):
+    """The reference schedule, gauge fixing and correlators at 64^3."""
+    import lceqcd_amd
+    from lceqcd_amd import couplings_from_xy
+    shape = tuple(int(x) for x in args.lattice.split(","))
+    V = shape[0] * shape[1] * shape[2]
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(9.0, *couplings_from_xy(9.0, 0.1, 0.5))
+    L.set_imode(0)
+    L.hotstart(314159, 0.5)
+    L.rng_philox(314159, 0)
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, args.warmup, measflag=1)
+    L.history(args.warmup)
+    if args.shape:
+        t, x = (int(v) for v in args.shape.split(","))
+        L.set_shape(t, bool(x))
+    else:  # tuned on the production sweep; the chain is restored
+        L.autotune(n_or=args.n_or, trials=10)
+    L.sync()
+    t0 = time.perf_counter()
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, args.steps, measflag=1)
+    L.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    hist = L.history(args.steps)
+    # agauge's KP launches and the ahiggs2/ahiggs3 launches, event-timed
+    L.profile_begin(lceqcd_amd.KC_LINK_KP, 6 * args.steps)
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, args.steps, measflag=1)
+    kp_ms, kp_n = L.profile_end()
+    L.history(args.steps)
+    L.profile_begin(lceqcd_amd.KC_HIGGS, 4 * args.steps)
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, args.steps, measflag=1)
+    hg_ms, hg_n = L.profile_end()
+    L.history(args.steps)
+    # Landau gauge fixing at the inputfile's parameters (f/landau_gauge.f)
+    L.set_gauge_fixing(1.7, 1.0, 1e-10)
+    L.sync()
+    t0 = time.perf_counter()
+    da, iters = L.landau_gauge()
+    gf_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    ncorr = 20
+    for _ in range(ncorr):
+        L.correlator()
+        L.scalarcorr()
+        L.polyakov()
+    corr_ms = (time.perf_counter() - t0) * 1e3 / ncorr
+    kp_us = kp_ms * 1e3 / max(kp_n, 1)
+    kp_bytes = LINK_PASS_BYTES_PER_SITE * V
+    # one iteration = two k_gfix launches (one per parity)
+    gf_us = gf_ms * 1e3 / max(iters, 1)
+    gf_gbps = 2 * GFIX_PASS_BYTES_PER_SITE * V / (gf_us * 1e-6) / 1e9
+    rec = {
+        "metric": METRIC,
+        "value": V / (ms * 1e-3),
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%dx%dx%d reference schedule ahiggs2 + ahiggs3 + agauge + renorm "
+                        "(beta=9, x=0.1, y=0.5, imode 0), hot start; Landau gauge fixing "
+                        "lambda=1.7 alpha=1 limit 1e-10; correlator + scalarcorr + polyakov"
+                        % shape,
+            "lattice": list(shape),
+            "sweep_algorithmic_GBps": REF_SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
+            "launch_shape": {"threads_per_block": L.get_shape()[0],
+                             "xcd_order": L.get_shape()[1]},
+            "higgs_launch_us": hg_ms * 1e3 / max(hg_n, 1),
+            "landau_gauge": {"iterations": iters, "da": da, "ms": gf_ms,
+                             "us_per_iteration": gf_us,
+                             "bytes_per_iteration": 2 * GFIX_PASS_BYTES_PER_SITE * V,
+                             "algorithmic_GBps": gf_gbps,
+                             "frac_of_hbm": gf_gbps / HBM_PEAK_GBPS},
+            "correlators_ms": corr_ms,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_link<true,false> (agauge: KP heatbath + hopping Metropolis)",
+            "achieved": kp_bytes / (kp_us * 1e-6) / 1e9,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": kp_bytes / (kp_us * 1e-6) / 1e9 / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": kp_bytes,
+            "avg_launch_us": kp_us,
+            "launches": kp_n,
+        },
+        "observables_last_sweep": {"plaq0": hist[-1, 0], "a02": hist[-1, 2],
+                                   "a04": hist[-1, 3], "a0kin": hist[-1, 1],
+                                   "link_accept": hist[-1, 7], "higgs_accept": hist[-1, 8]},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def learned(args, L0):
     import numpy as np
     import lceqcd_amd
@@ -278,7 +392,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
-                    choices=["single64", "replicas96", "mlp48", "mlp96"])
+                    choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64"])
     ap.add_argument("--sweeps", type=int, default=20000)
     ap.add_argument("--train-configs", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -296,6 +410,8 @@ def main():
         return calibrate_copy()
     if args.workload == "replicas96":
         return replicas96(args)
+    if args.workload == "refsched64":
+        return refsched64(args)
     if args.workload in ("mlp48", "mlp96"):
         return learned(args, int(args.workload[3:]))
 

@@ -96,7 +96,7 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   alloc(&L->partials, (size_t)LCEQ_NSLOT * LCEQ_NQ * L->pstride);
 
   alloc(&L->obs, LCEQ_OBS_N);
-  alloc(&L->slices, 14 * (size_t)lz);
+  alloc(&L->slices, 14 * (size_t)lz * corr_split(L->g));
   alloc(&L->corr, 5 * (size_t)lz + 8);
   L->hist_cap = 4096;
   alloc(&L->hist, (size_t)L->hist_cap * LCEQ_OBS_N);
@@ -618,15 +618,18 @@ extern "C" int lceq_landau_gauge(lceq_lattice* L, double* da1, int* iter_or1) {
   HIPCHK(hipMemcpyAsync(L->obs + OBS_GF_DONE, z2, 2 * sizeof(double), hipMemcpyHostToDevice,
                         L->stream));
   // Batches of guarded iterations; after convergence (or the cap) the
-  // remaining launches of a batch return at once.
-  int batch = 8;
+  // remaining launches of a batch return at once.  Inside a batch the loop
+  // condition for the iterations so far is tested by the next iteration's
+  // first launch; one finalize tests it after the batch's last iteration.
+  int batch = 8, done_iter = 0;
   double o[LCEQ_OBS_N];
   for (;;) {
     for (int k = 0; k < batch; ++k) {
-      launch_gfix(L, 0, 0, 0, 1);
+      launch_gfix(L, 0, 0, 0, 1, k > 0, done_iter);
       launch_gfix(L, 1, 1, 1, 1);
-      launch_gfix_finalize(L, 1);
+      ++done_iter;
     }
+    launch_gfix_finalize(L, 1, done_iter);
     int rc = check_launch();
     if (rc) return rc;
     if ((rc = read_obs(L, o))) return rc;

@@ -113,8 +113,13 @@ void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
                   uint64_t pass);
 void launch_renorm(lceq_lattice* L, double* U);
-void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded);
-void launch_gfix_finalize(lceq_lattice* L, int guarded);
+// One half-iteration of gauge fixing; check = 1 first tests the loop
+// condition for the `iter` iterations already complete (see k_gfix).
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int check = 0,
+                 int iter = 0);
+void launch_gfix_finalize(lceq_lattice* L, int guarded, int iter = 0);
+// blocks per z slice of the slice sums (the slices buffer holds 14 lz split)
+int corr_split(const Geom& g);
 void launch_slices(lceq_lattice* L);
 void launch_corr(lceq_lattice* L);
 void launch_heat_kpi(lceq_lattice* L, int n, const double* b, const double* a0_old, double* v,

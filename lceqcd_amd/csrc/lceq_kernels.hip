@@ -4,6 +4,8 @@
 // reference's MPI halo exchanges and barriers.  All arithmetic is FP64.
 #include <math.h>
 
+#include <algorithm>
+
 #include "lceq_internal.hpp"
 #include "lceq_sites.hpp"
 
@@ -321,27 +323,68 @@ struct GfArgs {
   double* Af;
   int eo, compute_da, renorm, guarded;
   double lambda, alpha;
-  const double* obs;  // OBS_GF_DONE read when guarded
+  double* obs;  // OBS_GF_DONE read when guarded; the convergence test writes
   double* part;
   int pstride, nb;
+  // convergence test of the previous iteration at the start of this launch
+  // (landau_gauge iterations after the first of a batch): `iter` iterations
+  // are complete, their last da is in `part`
+  int check;
+  double iter, limit_or, max_iter;
 };
+
+// da1 = (sum of the per-block partials of the eo = 1 half) / N
+// (f/or_gauge_fixing1.f:156-157), summed in a fixed order that does not
+// depend on the caller; every thread of a 256-thread block must call it.
+__device__ __forceinline__ double gfix_da(const double* __restrict__ part, int nb, double n) {
+  double v[1] = {0.0};
+  for (int i0 = threadIdx.x; i0 < nb; i0 += TB * 4) {
+    double y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) y[u] = i0 + TB * u < nb ? part[i0 + TB * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] += y[u];
+  }
+  block_sum<1>(v);
+  return v[0] / n;
+}
+
+// The loop condition of f/landau_gauge.f:54-59 after `iter` iterations
+// whose last da is `da`, plus the iteration cap: returns whether to stop;
+// one thread records da, the count and the flags.
+__device__ __forceinline__ bool gfix_stop(double da, double iter, double limit_or,
+                                          double max_iter, double* __restrict__ obs, bool writer) {
+  bool conv = !(da > limit_or);
+  bool stop = conv || iter >= max_iter;
+  if (writer) {
+    obs[OBS_DA] = da;
+    obs[OBS_ITER] = iter;
+    if (stop) obs[OBS_GF_DONE] = 1.0;
+    if (stop && !conv) obs[OBS_GF_CAPPED] = 1.0;
+  }
+  return stop;
+}
 
 // One parity half of f/or_gauge_fixing1.f:54-328.  Thread x reads and
 // writes only U_mu(x) and U_mu(x-mu) for mu = 1..3 and a_f(x), so the half
 // step has no cross-thread hazards.  With renorm=1 (eo = 1 of a
 // landau_gauge iteration) every link of the lattice is written exactly once
-// and f/u_f_renorm.f is applied on that write.
+// and f/u_f_renorm.f is applied on that write.  With check=1 every block
+// first repeats the convergence test of the previous iteration (the same
+// sum in the same order, so all blocks agree) and the launch returns when
+// the loop has ended; the operand loads are issued before the test.
 __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
   if (p.guarded && p.obs[OBS_GF_DONE] != 0.0) return;  // uniform over the grid
   const Geom& g = p.g;
   int blk = xcd_block(p.nb);
   int s = blk * TB + threadIdx.x;
-  double red[1] = {0.0};
-  if (s < g.n) {
-    int eo = p.eo, oe = eo ^ 1;
+  bool act = s < g.n;
+  int eo = p.eo, oe = eo ^ 1;
+  su2 ux[3], uy[3];
+  int sy[3];
+  vec3 af;
+  if (act) {
     Coord c = site_coord(g, eo, s);
-    su2 ux[3], uy[3];
-    int sy[3];
 #pragma unroll
     for (int mu = 0; mu < 3; ++mu) {
       int mx, my, mz;
@@ -350,6 +393,15 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       ux[mu] = ld_link(p.Uf, g, mu, eo, s);
       uy[mu] = ld_link(p.Uf, g, mu, oe, sy[mu]);
     }
+    af = ld_higgs(p.Af, g, eo, s);
+  }
+  if (p.check) {
+    double da = gfix_da(p.part, p.nb, (double)g.n);
+    if (gfix_stop(da, p.iter, p.limit_or, p.max_iter, p.obs, blk == 0 && threadIdx.x == 0))
+      return;  // block-uniform
+  }
+  double red[1] = {0.0};
+  if (act) {
     // :97-107, :116-142
     su2 s1{c_add(ux[0].a, ux[1].a), c_add(ux[0].b, ux[1].b)};
     s1.a = c_add(s1.a, c_scale(p.alpha, ux[2].a));
@@ -374,8 +426,7 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       o.a = c_mk(cos(tho), -t2);
       o.b = c_mk(-(f * o.b.re), -(f * o.b.im));
     }
-    vec3 af = ld_higgs(p.Af, g, eo, s);  // :243-252
-    su2 t = m_nn(o, m_ad(af, o));
+    su2 t = m_nn(o, m_ad(af, o));  // :243-252
     st_higgs(p.Af, g, eo, s, vec3{{t.b.im, t.b.re, t.a.im}});
 #pragma unroll
     for (int mu = 0; mu < 3; ++mu) {  // :256-324
@@ -392,39 +443,23 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
   if (p.compute_da) block_sum_store<1>(red, p.part, p.pstride, blk);
 }
 
-// da1 = sum/N (f/or_gauge_fixing1.f:156-157); for guarded (landau_gauge)
-// iterations also count the iteration and set the convergence flag
-// (f/landau_gauge.f:54-59) so that the rest of the batch exits at once.
+// da1 after one or_gauge_fixing call (guarded = 0), or the convergence test
+// of a landau_gauge batch's last iteration (guarded = 1, `iter` iterations
+// complete): the loop ends when da1 <= limit_or or at the iteration cap.
 __global__ __launch_bounds__(256) void k_gfix_finalize(const double* __restrict__ part, int nb,
-                                                       double n, int guarded, double limit_or,
-                                                       double max_iter, double* __restrict__ obs) {
+                                                       double n, int guarded, double iter,
+                                                       double limit_or, double max_iter,
+                                                       double* __restrict__ obs) {
   if (guarded && obs[OBS_GF_DONE] != 0.0) return;
-  double v[1] = {0.0};
-  for (int i0 = threadIdx.x; i0 < nb; i0 += TB * 4) {
-    double y[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) y[u] = i0 + TB * u < nb ? part[i0 + TB * u] : 0.0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[0] += y[u];
-  }
-  block_sum<1>(v);
-  if (threadIdx.x == 0) {
-    double da = v[0] / n;
+  double da = gfix_da(part, nb, n);
+  if (guarded)
+    gfix_stop(da, iter, limit_or, max_iter, obs, threadIdx.x == 0);
+  else if (threadIdx.x == 0)
     obs[OBS_DA] = da;
-    if (guarded) {
-      double it = obs[OBS_ITER] + 1.0;
-      obs[OBS_ITER] = it;
-      if (!(da > limit_or)) {
-        obs[OBS_GF_DONE] = 1.0;
-      } else if (it >= max_iter) {
-        obs[OBS_GF_DONE] = 1.0;
-        obs[OBS_GF_CAPPED] = 1.0;
-      }
-    }
-  }
 }
 
-void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded) {
+void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int check,
+                 int iter) {
   GfArgs a;
   a.g = L->g;
   a.Uf = L->Uf;
@@ -439,34 +474,43 @@ void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarde
   a.part = L->partials;
   a.pstride = L->pstride;
   a.nb = blocks_for(L->g.n);
+  a.check = check;
+  a.iter = (double)iter;
+  a.limit_or = L->limit_or;
+  a.max_iter = (double)L->max_iter;
   prof_begin(L, KC_GFIX);
   prof_count(L, KC_GFIX);
   hipLaunchKernelGGL(k_gfix, dim3(a.nb), dim3(TB), 0, L->stream, a);
   prof_end(L, KC_GFIX);
 }
 
-void launch_gfix_finalize(lceq_lattice* L, int guarded) {
+void launch_gfix_finalize(lceq_lattice* L, int guarded, int iter) {
   hipLaunchKernelGGL(k_gfix_finalize, dim3(1), dim3(TB), 0, L->stream, L->partials,
-                     blocks_for(L->g.n), (double)L->g.n, guarded, L->limit_or,
+                     blocks_for(L->g.n), (double)L->g.n, guarded, (double)iter, L->limit_or,
                      (double)L->max_iter, L->obs);
 }
 
 // ========================================================== correlators
 
 // z-slice sums over both parities (f/correlator.f:53-79, f/scalarcorr.f:43-56,
-// f/polyakov.f:172-185).  Quantities q (slices[q*lz + z]):
+// f/polyakov.f:172-185), each slice split over `split` blocks (about 512
+// sites each, corr_split).  Quantities q (partial j at
+// slices[(q*lz + z)*split + j]):
 //   0-2 a_f(1..3); 3+3*mu: Im alpha_mu, 4+3*mu: Im beta_mu, 5+3*mu: -Re beta_mu
 //   (the (Im a, -i b) projection of u_f); 12 rho(a); 13 cos(4|a|).
-__global__ __launch_bounds__(256) void k_slices(Geom g, const double* __restrict__ Uf,
+__global__ __launch_bounds__(256) void k_slices(Geom g, int split,
+                                                const double* __restrict__ Uf,
                                                 const double* __restrict__ Af,
                                                 const double* __restrict__ A,
                                                 double* __restrict__ slices) {
   __shared__ double red[4][14];
-  int z = blockIdx.x;
+  int z = blockIdx.x, j = blockIdx.y;
+  int nz = 2 * g.h2, chunk = (nz + split - 1) / split;
+  int i0 = j * chunk, i1 = min(nz, i0 + chunk);
   double acc[14];
 #pragma unroll
   for (int q = 0; q < 14; ++q) acc[q] = 0.0;
-  for (int i = threadIdx.x; i < 2 * g.h2; i += TB) {
+  for (int i = i0 + threadIdx.x; i < i1; i += TB) {
     int p = i >= g.h2, s = z * g.h2 + (i - p * g.h2);
     vec3 af = ld_higgs(Af, g, p, s), a = ld_higgs(A, g, p, s);
 #pragma unroll
@@ -492,80 +536,112 @@ __global__ __launch_bounds__(256) void k_slices(Geom g, const double* __restrict
   __syncthreads();
   if (threadIdx.x < 14) {
     int q = threadIdx.x;
-    slices[q * g.lz + z] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    slices[(q * g.lz + z) * split + j] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
   }
 }
 
 // Periodic z-correlators on the slice means (f/correlator.f:139-164,
 // f/scalarcorr.f:96-104, f/polyakov.f:225-233).  corr layout:
-// [0,lz) corr_a, [lz,2lz) corr_u, [2lz,3lz) corr_3, [3lz,4lz) rho corr,
-// [4lz,5lz) polyakov corr, 5lz: mean rho, 5lz+1: mean(P)^2, 5lz+2..4: a0f.
-__global__ __launch_bounds__(256) void k_corr(Geom g, double* __restrict__ slices,
-                                              double* __restrict__ corr) {
+// [0,lz) C_a, [lz,2lz) C_u, [2lz,3lz) C_3, [3lz,4lz) rho, [4lz,5lz) Polyakov,
+// then mean(rho), mean(P)^2, a0f(1..3).  The slice partials are summed in a
+// fixed order into slice means, held in LDS (dynamic, 14 lz doubles) when
+// LDS is set, else in `slices` itself.
+constexpr int CORR_TB = 1024;    // threads of the one k_corr block
+constexpr int CORR_MAX_SPLIT = 16;
+
+template <bool LDS>
+__global__ __launch_bounds__(CORR_TB) void k_corr(Geom g, int split, double* __restrict__ slices,
+                                                  double* __restrict__ corr) {
+  extern __shared__ double lds_slices[];
   int lz = g.lz;
   double area = (double)(g.lx * g.ly);
-  for (int i = threadIdx.x; i < 14 * lz; i += TB) slices[i] = slices[i] / area;
+  double* as = LDS ? lds_slices : slices;
+  // in place (LDS = false) is safe: element i reads only its own partials,
+  // and split = 1 then (see corr_split), so slices[i * split] = slices[i].
+  // All partials of an element are loaded before they are added (in order).
+  for (int i = threadIdx.x; i < 14 * lz; i += CORR_TB) {
+    double y[CORR_MAX_SPLIT];
+#pragma unroll
+    for (int j = 0; j < CORR_MAX_SPLIT; ++j) y[j] = j < split ? slices[i * split + j] : 0.0;
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < CORR_MAX_SPLIT; ++j)
+      if (j < split) v += y[j];
+    as[i] = v / area;
+  }
   __syncthreads();
-  const double* as = slices;
-  for (int z = threadIdx.x; z < lz; z += TB) {
-    double ca = 0.0, cu = 0.0, c3 = 0.0, cr = 0.0, cp = 0.0;
+  // one thread per (correlator, separation); each sum runs over x0 in order
+  for (int task = threadIdx.x; task < 5 * lz; task += CORR_TB) {
+    int kind = task / lz, z = task - kind * lz;
+    double c = 0.0;
     for (int x0 = 0; x0 < lz; ++x0) {
       int x1 = x0 + z;
       if (x1 >= lz) x1 -= lz;
-      double d = 0.0;
+      if (kind == 0) {
+        double d = 0.0;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) d = d + as[k * lz + x0] * as[k * lz + x1];
-      ca = ca + d;
-      // Re[A1 A1' + A2 conj(A2')] for mu = 1,2 and mu = 3, A1 real
-      double t = 0.0;
+        for (int k = 0; k < 3; ++k) d = d + as[k * lz + x0] * as[k * lz + x1];
+        c = c + d;
+      } else if (kind == 1) {
+        // Re[A1 A1' + A2 conj(A2')] for mu = 1,2, A1 real
+        double t = 0.0;
 #pragma unroll
-      for (int mu = 0; mu < 2; ++mu) {
-        double a1 = as[(3 + 3 * mu) * lz + x0], a1p = as[(3 + 3 * mu) * lz + x1];
-        double br = as[(4 + 3 * mu) * lz + x0], bi = as[(5 + 3 * mu) * lz + x0];
-        double brp = as[(4 + 3 * mu) * lz + x1], bip = as[(5 + 3 * mu) * lz + x1];
-        t = t + a1 * a1p;
-        t = t + (br * brp + bi * bip);
-      }
-      cu = cu + t;
-      {
+        for (int mu = 0; mu < 2; ++mu) {
+          double a1 = as[(3 + 3 * mu) * lz + x0], a1p = as[(3 + 3 * mu) * lz + x1];
+          double br = as[(4 + 3 * mu) * lz + x0], bi = as[(5 + 3 * mu) * lz + x0];
+          double brp = as[(4 + 3 * mu) * lz + x1], bip = as[(5 + 3 * mu) * lz + x1];
+          t = t + a1 * a1p;
+          t = t + (br * brp + bi * bip);
+        }
+        c = c + t;
+      } else if (kind == 2) {  // mu = 3
         double a1 = as[9 * lz + x0], a1p = as[9 * lz + x1];
         double br = as[10 * lz + x0], bi = as[11 * lz + x0];
         double brp = as[10 * lz + x1], bip = as[11 * lz + x1];
-        c3 = c3 + (a1 * a1p + (br * brp + bi * bip));
+        c = c + (a1 * a1p + (br * brp + bi * bip));
+      } else {
+        int q = kind == 3 ? 12 : 13;
+        c = c + as[q * lz + x0] * as[q * lz + x1];
       }
-      cr = cr + as[12 * lz + x0] * as[12 * lz + x1];
-      cp = cp + as[13 * lz + x0] * as[13 * lz + x1];
     }
-    corr[z] = ca / lz;
-    corr[lz + z] = cu / (lz * 2);
-    corr[2 * lz + z] = c3 / lz;
-    corr[3 * lz + z] = cr / lz;
-    corr[4 * lz + z] = cp / lz;
+    corr[kind * lz + z] = kind == 1 ? c / (lz * 2) : c / lz;
   }
-  if (threadIdx.x == 0) {
-    double mr = 0.0, mp = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    for (int z = 0; z < lz; ++z) {
-      mr += as[12 * lz + z];
-      mp += as[13 * lz + z];
-      f0 += as[z];
-      f1 += as[lz + z];
-      f2 += as[2 * lz + z];
-    }
-    mp = mp / lz;
-    corr[5 * lz] = mr / lz;
-    corr[5 * lz + 1] = mp * mp;
-    corr[5 * lz + 2] = f0 / lz;  // a0f(i) = mean over sites = mean of slice means
-    corr[5 * lz + 3] = f1 / lz;
-    corr[5 * lz + 4] = f2 / lz;
+  // the means, one quantity per thread, summed over z in order
+  if (threadIdx.x < 5) {
+    const int qs[5] = {12, 13, 0, 1, 2};
+    int q = qs[threadIdx.x];
+    double m = 0.0;
+    for (int z = 0; z < lz; ++z) m += as[q * lz + z];
+    m = m / lz;
+    // mean(rho), mean(P)^2, a0f(1..3) = mean over sites = mean of slice means
+    corr[5 * lz + threadIdx.x] = threadIdx.x == 1 ? m * m : m;
   }
 }
 
+// slice means in LDS up to this many bytes (one block, 64 KB per workgroup)
+constexpr size_t CORR_LDS_MAX = 64 * 1024;
+
+int corr_split(const Geom& g) {
+  if (14 * (size_t)g.lz * sizeof(double) > CORR_LDS_MAX) return 1;
+  int nz = 2 * g.h2;
+  // 512 sites per block, at most CORR_MAX_SPLIT blocks per slice
+  return std::max(1, std::min(CORR_MAX_SPLIT, (nz + 2 * TB - 1) / (2 * TB)));
+}
+
 void launch_slices(lceq_lattice* L) {
-  hipLaunchKernelGGL(k_slices, dim3(L->g.lz), dim3(TB), 0, L->stream, L->g, L->Uf, L->Af, L->A,
-                     L->slices);
+  int split = corr_split(L->g);
+  hipLaunchKernelGGL(k_slices, dim3(L->g.lz, split), dim3(TB), 0, L->stream, L->g, split, L->Uf,
+                     L->Af, L->A, L->slices);
 }
 void launch_corr(lceq_lattice* L) {
-  hipLaunchKernelGGL(k_corr, dim3(1), dim3(TB), 0, L->stream, L->g, L->slices, L->corr);
+  int split = corr_split(L->g);
+  size_t lds = 14 * (size_t)L->g.lz * sizeof(double);
+  if (lds <= CORR_LDS_MAX)
+    hipLaunchKernelGGL(k_corr<true>, dim3(1), dim3(CORR_TB), lds, L->stream, L->g, split,
+                       L->slices, L->corr);
+  else
+    hipLaunchKernelGGL(k_corr<false>, dim3(1), dim3(CORR_TB), 0, L->stream, L->g, split,
+                       L->slices, L->corr);
 }
 
 // ============================================ array-level entry points
