@@ -574,16 +574,21 @@ __global__ __launch_bounds__(CORR_TB) void k_corr(Geom g, int split, double* __r
   for (int task = threadIdx.x; task < 5 * lz; task += CORR_TB) {
     int kind = task / lz, z = task - kind * lz;
     double c = 0.0;
-    for (int x0 = 0; x0 < lz; ++x0) {
-      int x1 = x0 + z;
-      if (x1 >= lz) x1 -= lz;
-      if (kind == 0) {
+    // one loop per kind, unrolled so that the LDS reads of several x0 are in
+    // flight together; the additions stay in x0 order
+    if (kind == 0) {
+#pragma unroll 8
+      for (int x0 = 0; x0 < lz; ++x0) {
+        int x1 = x0 + z < lz ? x0 + z : x0 + z - lz;
         double d = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) d = d + as[k * lz + x0] * as[k * lz + x1];
         c = c + d;
-      } else if (kind == 1) {
-        // Re[A1 A1' + A2 conj(A2')] for mu = 1,2, A1 real
+      }
+    } else if (kind == 1) {  // Re[A1 A1' + A2 conj(A2')] for mu = 1,2, A1 real
+#pragma unroll 8
+      for (int x0 = 0; x0 < lz; ++x0) {
+        int x1 = x0 + z < lz ? x0 + z : x0 + z - lz;
         double t = 0.0;
 #pragma unroll
         for (int mu = 0; mu < 2; ++mu) {
@@ -594,14 +599,22 @@ __global__ __launch_bounds__(CORR_TB) void k_corr(Geom g, int split, double* __r
           t = t + (br * brp + bi * bip);
         }
         c = c + t;
-      } else if (kind == 2) {  // mu = 3
+      }
+    } else if (kind == 2) {  // mu = 3
+#pragma unroll 8
+      for (int x0 = 0; x0 < lz; ++x0) {
+        int x1 = x0 + z < lz ? x0 + z : x0 + z - lz;
         double a1 = as[9 * lz + x0], a1p = as[9 * lz + x1];
         double br = as[10 * lz + x0], bi = as[11 * lz + x0];
         double brp = as[10 * lz + x1], bip = as[11 * lz + x1];
         c = c + (a1 * a1p + (br * brp + bi * bip));
-      } else {
-        int q = kind == 3 ? 12 : 13;
-        c = c + as[q * lz + x0] * as[q * lz + x1];
+      }
+    } else {  // rho, Polyakov
+      const double* q = as + (kind == 3 ? 12 : 13) * lz;
+#pragma unroll 8
+      for (int x0 = 0; x0 < lz; ++x0) {
+        int x1 = x0 + z < lz ? x0 + z : x0 + z - lz;
+        c = c + q[x0] * q[x1];
       }
     }
     corr[kind * lz + z] = kind == 1 ? c / (lz * 2) : c / lz;
@@ -611,6 +624,7 @@ __global__ __launch_bounds__(CORR_TB) void k_corr(Geom g, int split, double* __r
     const int qs[5] = {12, 13, 0, 1, 2};
     int q = qs[threadIdx.x];
     double m = 0.0;
+#pragma unroll 8
     for (int z = 0; z < lz; ++z) m += as[q * lz + z];
     m = m / lz;
     // mean(rho), mean(P)^2, a0f(1..3) = mean over sites = mean of slice means
