@@ -293,7 +293,7 @@ def learned(args, L0):
     L.set_couplings(*coup)
     L.hotstart(12345, 0.5)
     L.rng_philox(12345, 0)
-    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 300, n_or=4, delta=0.4, measflag=0)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, args.therm, n_or=4, delta=0.4, measflag=0)
     # training: 2000 configurations, a subsample of sites from each
     M = lceqcd_amd.Mlp(seed=1)
     stride = max(1, V // 2 // 3456)
@@ -345,7 +345,7 @@ def learned(args, L0):
         "unit": "independent samples/s of <Tr A^2>",
         "n_gpus": 1,
         "steps": args.sweeps,
-        "warmup": 300,
+        "warmup": args.therm,
         "ms_per_step": 1e3 / lrn["sweeps_per_s"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -396,6 +396,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=20000)
     ap.add_argument("--train-configs", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--therm", type=int, default=300,
+                    help="thermalisation sweeps before training (learned workloads)")
     ap.add_argument("--hist-out", default=None)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--shape", default=None,

@@ -263,6 +263,10 @@ int lceq_mlp_clear(lceq_mlp *M);
  * NULL) receives the mean negative log-likelihood of every step */
 int lceq_mlp_train(lceq_mlp *M, int steps, int batch, double lr, uint64_t seed, int fit_norm,
                    double *loss);
+/* launch shape of the learned Higgs update: sites per wave 64 (four MFMA
+ * column tiles, the default, also selected by 0) or 32.  Fields do not
+ * depend on it. */
+int lceq_mlp_set_sites_per_wave(lceq_mlp *M, int sites);
 /* the inference network on n rows of the 10 raw inputs: out[n][6] */
 int lceq_mlp_eval(lceq_mlp *M, const float *x, int n, float *out);
 /* one Higgs sweep (both parities) with the learned proposal; accept is the

@@ -47,17 +47,27 @@ constexpr int TB = 256;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx8 __attribute__((ext_vector_type(8)));
 
-// Inference weights in MFMA operand order.  B operand of
-// v_mfma_f32_16x16x32_bf16 (K = 32 x N = 16): lane l holds column l % 16,
-// rows 8 (l / 16) .. 8 (l / 16) + 7 of the K block.
+// Inference weights as A operands of v_mfma_f32_16x16x32_bf16 (M = 16
+// output units x K = 32 inputs): lane l holds row l % 16 (an output unit) and
+// K elements 8 (l / 16) .. + 7.  The sites are the N (column) dimension, so a
+// layer's result (D: lane l holds column l % 16, rows 4 (l / 16) .. + 3) is,
+// after bias and SiLU, the next layer's B operand without leaving the lane:
+// K element 8g + j of block kb is hidden unit 32 kb + 4g + j (j < 4) or
+// 32 kb + 16 + 4g + (j - 4) (j >= 4), the rows that D tiles 2 kb and
+// 2 kb + 1 give lane group g.  Layers 2 and 3 are packed in that K order.
 struct MlpPack {
   bf16x8 w1[4][64];     // [n tile][lane], K = 10 inputs padded to 32
-  bf16x8 w2[4][2][64];  // [n tile][k block][lane]
-  bf16x8 w3[2][64];     // [k block][lane], N = 6 outputs padded to 16
+  bf16x8 w2[4][2][64];  // [n tile][k block][lane], K in the order above
+  bf16x8 w3[2][64];     // [k block][lane], M = 6 outputs padded to 16
   float b1[NH], b2[NH], b3[16];
   float xmean[NIN], xinv[NIN];
 };
+// hidden unit of K element 8 g + j of block kb (layers 2 and 3)
+__host__ __device__ constexpr int hidden_of_k(int kb, int g, int j) {
+  return 32 * kb + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
 
 // ------------------------------------------------------------- frame + features
 
@@ -114,22 +124,16 @@ __host__ __device__ inline void raw_features(const double y[3], double fn, doubl
 
 // ----------------------------------------------------------------- inference
 
-// Per-wave scratch of one evaluation: bf16 inputs (row = lane), reused for
-// the fp32 outputs, and the bf16 hidden activations
+// Per-wave scratch of one evaluation: the sites' bf16 input rows (K padded
+// to 16), the fp32 outputs, and the biases in D-row order, the accumulator
+// operand of each MFMA chain: bias[4 nt + g] = units 16 nt + 4g .. + 3 of
+// b1, then b2 from 16, then b3 from 32
 struct alignas(16) MlpLds {
-  union {
-    unsigned short x[64 * 32];
-    float o[64 * 16];
-  };
-  unsigned short h[64 * 64];
+  bf16x8 x[64][2];
+  float o[64][8];
+  floatx4 bias[36];
 };
 
-__device__ __forceinline__ unsigned short to_bf16_bits(float v) {
-  __bf16 b = (__bf16)v;
-  unsigned short u;
-  memcpy(&u, &b, 2);
-  return u;
-}
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
 // inference SiLU: hardware exp and reciprocal (the activations are rounded
 // to bf16 next; exactness of the chain only needs a deterministic network)
@@ -141,104 +145,100 @@ __device__ __forceinline__ float silu_fast(float z) {
 // order, so the evaluation needs no block barrier.
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
-// This lane's share of the inference weights, loaded once per kernel so
-// that their latency hides under the site loads
+// This lane's share of the inference weights (A-operand fragments), held
+// in registers for the whole kernel: read from LDS per tile instead, they
+// cost more in latency than the registers they free
 struct MlpRegs {
   bf16x8 w1[4], w2[4][2], w3[2];
-  float b1[4], b2[4], b3;
 };
 
 __device__ __forceinline__ void mlp_load(const MlpPack* __restrict__ P, MlpRegs& R) {
-  const int l = threadIdx.x & 63, col = l & 15;
+  const int l = threadIdx.x & 63;
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
     R.w1[nt] = P->w1[nt][l];
     R.w2[nt][0] = P->w2[nt][0][l];
     R.w2[nt][1] = P->w2[nt][1][l];
-    R.b1[nt] = P->b1[16 * nt + col];
-    R.b2[nt] = P->b2[16 * nt + col];
   }
   R.w3[0] = P->w3[0][l];
   R.w3[1] = P->w3[1][l];
-  R.b3 = P->b3[col];
 }
 
-// One evaluation for the 64 sites of this wave (row = lane).  Every lane of
-// the wave must call it.
+// The biases into this wave's scratch (once per kernel, before mlp_eval)
+__device__ __forceinline__ void mlp_stage_bias(const MlpPack* __restrict__ P, MlpLds& S) {
+  const int l = threadIdx.x & 63;
+  if (l < 16)
+    S.bias[l] = *reinterpret_cast<const floatx4*>(&P->b1[4 * l]);
+  else if (l < 32)
+    S.bias[l] = *reinterpret_cast<const floatx4*>(&P->b2[4 * (l - 16)]);
+  else if (l < 36)
+    S.bias[l] = *reinterpret_cast<const floatx4*>(&P->b3[4 * (l - 32)]);
+  wave_sync();
+}
+
+// SiLU of two D tiles (units 32 kb + 4g + i and 32 kb + 16 + 4g + i, bias
+// included) -> the bf16 B operand of K block kb
+__device__ __forceinline__ bf16x8 silu_pack(const floatx4& d0, const floatx4& d1) {
+  floatx8 v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[i] = silu_fast(d0[i]);
+    v[4 + i] = silu_fast(d1[i]);
+  }
+  return __builtin_convertvector(v, bf16x8);
+}
+
+// One evaluation for the 16 ST sites of this wave (lane l < 16 ST = site;
+// the other lanes' outputs are zero).  Every lane of the wave must call it.
+// The sites are the MFMA columns: ST tiles of 16.
+template <int ST>
 __device__ void mlp_eval(const MlpPack* __restrict__ P, const MlpRegs& R, const float (&xin)[NIN],
                          float (&out)[NOUT], MlpLds& S) {
-  const int l = threadIdx.x & 63, col = l & 15, grp = l >> 4;
-  // inputs, normalised, as a bf16 row padded to 32: four 16-byte stores
-  unsigned short row[32];
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  // this site's inputs, normalised, as a bf16 row of 16 (10 used)
+  floatx8 f0, f1;
 #pragma unroll
-  for (int k = 0; k < 32; ++k)
-    row[k] = k < NIN ? to_bf16_bits((xin[k] - P->xmean[k]) * P->xinv[k]) : (unsigned short)0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    bf16x8 v;
-    memcpy(&v, &row[8 * q], 16);
-    *reinterpret_cast<bf16x8*>(&S.x[l * 32 + 8 * q]) = v;
+  for (int k = 0; k < 8; ++k) {
+    f0[k] = (xin[k] - P->xmean[k]) * P->xinv[k];
+    f1[k] = 8 + k < NIN ? (xin[8 + k] - P->xmean[8 + k]) * P->xinv[8 + k] : 0.0f;
+  }
+  if (l < 16 * ST) {
+    S.x[l][0] = __builtin_convertvector(f0, bf16x8);
+    S.x[l][1] = __builtin_convertvector(f1, bf16x8);
   }
   wave_sync();
-  // layer 1: [64 x 32] x [32 x 64]
-  floatx4 acc[4][4];
+  const bf16x8 zero8 = {};
+  // one site tile (16 columns) at a time through the three layers: the
+  // tiles are independent, and this keeps one tile's activations live
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    bf16x8 a = *reinterpret_cast<const bf16x8*>(&S.x[(16 * mt + col) * 32 + 8 * grp]);
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, R.w1[nt], floatx4{0, 0, 0, 0}, 0, 0, 0);
-  }
-  // D of a 16 x 16 tile: lane l holds column l % 16, rows 4 (l / 16) .. + 3
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
+  for (int st = 0; st < ST; ++st) {
+    // B operand of layer 1: site 16 st + c, inputs 8g .. 8g + 7 (zero for g >= 2)
+    bf16x8 xb = g < 2 ? S.x[16 * st + c][g] : zero8;
+    // layer 1: D[unit][site] = W1^T [64 x 32] X^T [32 x 16]
+    floatx4 acc[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int m = 16 * mt + 4 * grp + i, n = 16 * nt + col;
-        S.h[m * 64 + n] = to_bf16_bits(silu_fast(acc[mt][nt][i] + R.b1[nt]));
-      }
-  wave_sync();
-  // layer 2: [64 x 64] x [64 x 64]
-  bf16x8 a2[4][2];
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-      a2[mt][kb] = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 32 * kb + 8 * grp]);
-  wave_sync();
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[nt], xb, S.bias[4 * nt + g], 0, 0, 0);
+    bf16x8 h0 = silu_pack(acc[0], acc[1]);
+    bf16x8 h1 = silu_pack(acc[2], acc[3]);
+    // layer 2: [64 x 64] x [64 x 16]
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][0], R.w2[nt][0],
-                                                          floatx4{0, 0, 0, 0}, 0, 0, 0);
-      acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[mt][1], R.w2[nt][1], c, 0, 0, 0);
+      floatx4 t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[nt][0], h0, S.bias[16 + 4 * nt + g],
+                                                          0, 0, 0);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[nt][1], h1, t, 0, 0, 0);
     }
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int m = 16 * mt + 4 * grp + i, n = 16 * nt + col;
-        S.h[m * 64 + n] = to_bf16_bits(silu_fast(acc[mt][nt][i] + R.b2[nt]));
-      }
-  wave_sync();
-  // layer 3: [64 x 64] x [64 x 16]
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    bf16x8 x0 = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 8 * grp]);
-    bf16x8 x1 = *reinterpret_cast<const bf16x8*>(&S.h[(16 * mt + col) * 64 + 32 + 8 * grp]);
-    floatx4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x0, R.w3[0], floatx4{0, 0, 0, 0}, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x1, R.w3[1], c, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) S.o[(16 * mt + 4 * grp + i) * 16 + col] = c[i] + R.b3;
+    h0 = silu_pack(acc[0], acc[1]);
+    h1 = silu_pack(acc[2], acc[3]);
+    // layer 3: [16 (6 used) x 64] x [64 x 16]; lane groups 0 and 1 hold the
+    // outputs 0-3 and 4-7 of site 16 st + c
+    floatx4 t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w3[0], h0, S.bias[32 + g], 0, 0, 0);
+    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w3[1], h1, t, 0, 0, 0);
+    if (g < 2) *reinterpret_cast<floatx4*>(&S.o[16 * st + c][4 * g]) = t;
   }
   wave_sync();
 #pragma unroll
-  for (int j = 0; j < NOUT; ++j) out[j] = S.o[l * 16 + j];
+  for (int j = 0; j < NOUT; ++j) out[j] = l < 16 * ST ? S.o[l][j] : 0.0f;
   wave_sync();
 }
 
@@ -281,68 +281,89 @@ struct MlpArgs {
 // stream 0 of this pass): k = 0, 1 and 2, 3 give the three Gaussians by
 // Box-Muller, k = 4 the accept.  Partials: accepts, rho, rho^2, a.F (the
 // update always measures).
-__global__ __launch_bounds__(256) void k_higgs_mlp(MlpArgs p) {
+template <int ST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_higgs_mlp(MlpArgs p) {
   __shared__ MlpLds lds[4];
   MlpLds& S = lds[threadIdx.x >> 6];
+  mlp_stage_bias(p.pack, S);
   int blk = xcd_block(p.nb);
-  unsigned s = (unsigned)(blk * TB + threadIdx.x);
-  bool valid = s < (unsigned)p.g.n;
+  // 16 ST sites per wave, lane l < 16 ST owns site (wave) 16 ST + l
+  int lane = threadIdx.x & 63;
+  unsigned s = (unsigned)((blk * (TB / 64) + (threadIdx.x >> 6)) * 16 * ST + lane);
+  bool valid = lane < 16 * ST && s < (unsigned)p.g.n;
   unsigned sl = valid ? s : 0u;
   const Geom& g = p.g;
-  MlpRegs W;
-  mlp_load(p.pack, W);
   Coord c = site_coord(g, p.eo, (int)sl);
   ForceLinks k;
   load_force_links(p.U, p.A, g, p.eo, sl, c, k);
   vec3 av = ld_higgs(p.A, g, p.eo, sl);
   __builtin_amdgcn_sched_barrier(0);
-  double u0, u1, u2, u3, racc, dummy;
-  philox_pair(p.key, sl, 0, p.pass, 0, u0, u1);
-  philox_pair(p.key, sl, 0, p.pass, 1, u2, u3);
-  philox_pair(p.key, sl, 0, p.pass, 2, racc, dummy);
+  // under the load latency: the uniforms and the Box-Muller Gaussians
+  double z[3], racc;
+  {
+    double u0, u1, u2, u3, dummy;
+    philox_pair(p.key, sl, 0, p.pass, 0, u0, u1);
+    philox_pair(p.key, sl, 0, p.pass, 1, u2, u3);
+    philox_pair(p.key, sl, 0, p.pass, 2, racc, dummy);
+    double r01 = sqrt(-2.0 * log(u0)), r23 = sqrt(-2.0 * log(u2));
+    z[0] = r01 * cos(TPI * u1);
+    z[1] = r01 * sin(TPI * u1);
+    z[2] = r23 * cos(TPI * u3);
+  }
   vec3 F = force_sum(k);
   Frame fr = make_frame(F.x);
-  double ya[3];
-  to_frame(fr, av.x, ya);
-  float xin[NIN], o1[NOUT], o2[NOUT];
-  raw_features(ya, fr.fn, p.beta_a, p.beta_2, p.beta_4, xin);
-  mlp_eval(p.pack, W, xin, o1, S);
-  double mu[3], sig[3], ls[3];
-  proposal(o1, mu, sig, ls);
-  // Box-Muller
-  double r01 = sqrt(-2.0 * log(u0)), r23 = sqrt(-2.0 * log(u2));
-  double z[3] = {r01 * cos(TPI * u1), r01 * sin(TPI * u1), r23 * cos(TPI * u3)};
-  double yp[3], an[3], ypr[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) yp[i] = mu[i] + sig[i] * z[i];
-  from_frame(fr, yp, an);
-  to_frame(fr, an, ypr);
-  raw_features(ypr, fr.fn, p.beta_a, p.beta_2, p.beta_4, xin);
-  mlp_eval(p.pack, W, xin, o2, S);
-  double mu2[3], sig2[3], ls2[3];
-  proposal(o2, mu2, sig2, ls2);
-  double lqf = log_q(ypr, mu, sig, ls), lqr = log_q(ya, mu2, sig2, ls2);
-  // site action difference as f/ahiggs1.f:64-92
-  double sfo = 0.0, sfn = 0.0, ro = 0.0, rn = 0.0;
+  // site action terms of the current a, f/ahiggs1.f:64-92
+  double sfo = 0.0, ro = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     sfo = sfo + av.x[i] * F.x[i];
-    sfn = sfn + an[i] * F.x[i];
     ro = ro + av.x[i] * av.x[i];
+  }
+  double ya[3], fn = fr.fn;
+  to_frame(fr, av.x, ya);
+  float xin[NIN], o[NOUT];
+  raw_features(ya, fn, p.beta_a, p.beta_2, p.beta_4, xin);
+  MlpRegs W;
+  mlp_load(p.pack, W);
+  mlp_eval<ST>(p.pack, W, xin, o, S);
+  // forward proposal a' and log q(a' | a); only these and the action terms
+  // stay live across the second evaluation
+  double an[3], ypr[3], lqf, sfn = 0.0, rn = 0.0;
+  {
+    double mu[3], sig[3], ls[3], yp[3];
+    proposal(o, mu, sig, ls);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) yp[i] = mu[i] + sig[i] * z[i];
+    from_frame(fr, yp, an);
+    to_frame(fr, an, ypr);
+    lqf = log_q(ypr, mu, sig, ls);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    sfn = sfn + an[i] * F.x[i];
     rn = rn + an[i] * an[i];
+  }
+  raw_features(ypr, fn, p.beta_a, p.beta_2, p.beta_4, xin);
+  mlp_eval<ST>(p.pack, W, xin, o, S);
+  double lqr;
+  {
+    double mu2[3], sig2[3], ls2[3];
+    proposal(o, mu2, sig2, ls2);
+    lqr = log_q(ya, mu2, sig2, ls2);
   }
   double boltz = exp(-(p.beta_a * (sfo - sfn)) + p.beta_2 * (ro - rn) +
                      p.beta_4 * (ro * ro - rn * rn) + (lqr - lqf));
   double red[4] = {0.0, 0.0, 0.0, 0.0};
+  double af = sfo;  // a.F of the site after the update
   if (boltz >= racc) {
-    av = vec3{{an[0], an[1], an[2]}};
+    if (valid) st_higgs(p.A, g, p.eo, s, vec3{{an[0], an[1], an[2]}});
     ro = rn;
-    if (valid) st_higgs(p.A, g, p.eo, s, av);
+    af = sfn;
     red[0] = 1.0;
   }
   red[1] = ro;
   red[2] = ro * ro;
-  red[3] = (av.x[0] * F.x[0] + av.x[1] * F.x[1]) + av.x[2] * F.x[2];
+  red[3] = af;
   if (!valid) red[0] = red[1] = red[2] = red[3] = 0.0;
   block_sum_store<4>(red, p.part, p.pstride, blk);
 }
@@ -357,9 +378,10 @@ __global__ __launch_bounds__(256) void k_mlp_eval(const MlpPack* __restrict__ P,
   float xin[NIN], o[NOUT];
   MlpRegs W;
   mlp_load(P, W);
+  mlp_stage_bias(P, lds[threadIdx.x >> 6]);
 #pragma unroll
   for (int k = 0; k < NIN; ++k) xin[k] = x[(size_t)il * NIN + k];
-  mlp_eval(P, W, xin, o, lds[threadIdx.x >> 6]);
+  mlp_eval<4>(P, W, xin, o, lds[threadIdx.x >> 6]);
   if (i < n)
 #pragma unroll
     for (int j = 0; j < NOUT; ++j) y[(size_t)i * NOUT + j] = o[j];
@@ -548,6 +570,7 @@ struct lceq_mlp {
   int* idx;
   int idx_cap;
   MlpPack* pack;
+  int sites_per_wave;  // learned Higgs launch: 0 = by lattice size, 32 or 64
   float* data;  // NREC floats per record
   int64_t nrec, cap;
   int64_t adam_t;
@@ -569,22 +592,25 @@ static int mlp_upload(lceq_mlp* M) {
   MlpPack h;
   memset(&h, 0, sizeof(h));
   const float* w = M->host.data();
-  auto frag = [&](bf16x8& f, int K, int N, int off, int kb, int nt, int l) {
+  // A operand of lane l: row (output unit) 16 nt + l % 16, K elements
+  // 8 (l / 16) + j, natural for the inputs, hidden_of_k for hidden layers
+  auto frag = [&](bf16x8& f, int K, int N, int off, int kb, int nt, int l, bool hidden) {
     unsigned short e[8];
     for (int j = 0; j < 8; ++j) {
-      int k = 32 * kb + 8 * (l / 16) + j, n = 16 * nt + l % 16;
+      int g = l / 16, n = 16 * nt + l % 16;
+      int k = hidden ? hidden_of_k(kb, g, j) : 32 * kb + 8 * g + j;
       e[j] = (k < K && n < N) ? host_bf16(w[off + k * N + n]) : (unsigned short)0;
     }
     memcpy(&f, e, 16);
   };
   for (int l = 0; l < 64; ++l) {
     for (int nt = 0; nt < 4; ++nt) {
-      frag(h.w1[nt][l], NIN, NH, OFF_W1, 0, nt, l);
-      frag(h.w2[nt][0][l], NH, NH, OFF_W2, 0, nt, l);
-      frag(h.w2[nt][1][l], NH, NH, OFF_W2, 1, nt, l);
+      frag(h.w1[nt][l], NIN, NH, OFF_W1, 0, nt, l, false);
+      frag(h.w2[nt][0][l], NH, NH, OFF_W2, 0, nt, l, true);
+      frag(h.w2[nt][1][l], NH, NH, OFF_W2, 1, nt, l, true);
     }
-    frag(h.w3[0][l], NH, NOUT, OFF_W3, 0, 0, l);
-    frag(h.w3[1][l], NH, NOUT, OFF_W3, 1, 0, l);
+    frag(h.w3[0][l], NH, NOUT, OFF_W3, 0, 0, l, true);
+    frag(h.w3[1][l], NH, NOUT, OFF_W3, 1, 0, l, true);
   }
   for (int j = 0; j < NH; ++j) {
     h.b1[j] = w[OFF_B1 + j];
@@ -814,6 +840,15 @@ extern "C" int lceq_mlp_train(lceq_mlp* M, int steps, int batch, double lr, uint
   return mlp_upload(M);
 }
 
+extern "C" int lceq_mlp_set_sites_per_wave(lceq_mlp* M, int sites) {
+  if (!M) return mfail(LCEQ_ERR_ARG, "null argument");
+  if (sites != 0 && sites != 32 && sites != 64)
+    return mfail(LCEQ_ERR_ARG, "sites per wave must be 0 (by lattice size), 32 or 64, not %d",
+                 sites);
+  M->sites_per_wave = sites;
+  return LCEQ_OK;
+}
+
 extern "C" int lceq_mlp_eval(lceq_mlp* M, const float* x, int n, float* y) {
   if (!M || n < 1 || !x || !y) return mfail(LCEQ_ERR_ARG, "bad arguments");
   float *dx = nullptr, *dy = nullptr;
@@ -842,6 +877,19 @@ extern "C" int lceq_mlp_eval(lceq_mlp* M, const float* x, int n, float* y) {
 
 // both parities with the learned proposal; partials (always measured) in the
 // Higgs slots
+// Sites per wave of the learned Higgs launch: 64 (four MFMA column tiles,
+// the default) or 32 (two tiles, half the lanes idle in the per-site FP64
+// work).  32 shortens one wave's network evaluations but not its FP64
+// chain; at 48^3 (fewer waves than SIMDs) it measured 15.6 us per parity
+// against 13.8 us for 64.  The fields do not depend on the choice.
+static int mlp_sites_per_wave(const lceq_lattice*, const lceq_mlp* M) {
+  return M->sites_per_wave ? M->sites_per_wave : 64;
+}
+static int mlp_blocks(const lceq_lattice* L, const lceq_mlp* M) {
+  int per_block = (TB / 64) * mlp_sites_per_wave(L, M);
+  return (L->g.n + per_block - 1) / per_block;
+}
+
 static int mlp_higgs_sweep(lceq_lattice* L, lceq_mlp* M) {
   if (L->rng_mode != LCEQ_RNG_PHILOX) return mfail(LCEQ_ERR_ARG, "the learned proposal draws Philox");
   MlpArgs a;
@@ -854,14 +902,18 @@ static int mlp_higgs_sweep(lceq_lattice* L, lceq_mlp* M) {
   a.key = Philox{(uint32_t)L->seed, (uint32_t)(L->seed >> 32)};
   a.pack = M->pack;
   a.pstride = L->pstride;
-  a.nb = blocks_for(L->g.n);
+  int spw = mlp_sites_per_wave(L, M);
+  a.nb = mlp_blocks(L, M);
   prof_begin(L, KC_HIGGS);
   for (int eo = 0; eo < 2; ++eo) {
     a.eo = eo;
     a.pass = next_pass(L);
     a.part = L->partials + (size_t)(6 + eo) * LCEQ_NQ * L->pstride;
     prof_count(L, KC_HIGGS);
-    hipLaunchKernelGGL(k_higgs_mlp, dim3(a.nb), dim3(TB), 0, L->stream, a);
+    if (spw == 32)
+      hipLaunchKernelGGL(k_higgs_mlp<2>, dim3(a.nb), dim3(TB), 0, L->stream, a);
+    else
+      hipLaunchKernelGGL(k_higgs_mlp<4>, dim3(a.nb), dim3(TB), 0, L->stream, a);
   }
   prof_end(L, KC_HIGGS);
   return check_launch();
@@ -871,7 +923,7 @@ extern "C" int lceq_higgs_mlp(lceq_lattice* L, lceq_mlp* M, double* accept) {
   if (!L || !M) return mfail(LCEQ_ERR_ARG, "null argument");
   int rc = mlp_higgs_sweep(L, M);
   if (rc) return rc;
-  launch_finalize(L, 0, 0, 2, 1, nullptr, blocks_for(L->g.n));
+  launch_finalize(L, 0, 0, 2, 1, nullptr, mlp_blocks(L, M));
   if (accept) {
     double o[LCEQ_OBS_N];
     if ((rc = read_obs(L, o))) return rc;
@@ -897,7 +949,7 @@ extern "C" int lceq_sweep_mlp(lceq_lattice* L, lceq_mlp* M, int nsweep, int n_or
       if ((rc = link_sweep(L, 0, m, 1, m))) return rc;
     }
     if ((rc = mlp_higgs_sweep(L, M))) return rc;
-    launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 2, 1, row, blocks_for(L->g.n));
+    launch_finalize(L, measflag ? (n_or ? 2 : 1) : 0, measflag, 2, 1, row, mlp_blocks(L, M));
   }
   return check_launch();
 }

@@ -65,6 +65,11 @@ class Mlp:
                                        loss.ctypes.data_as(C.POINTER(C.c_double))))
         return loss[:steps]
 
+    def set_sites_per_wave(self, sites=0):
+        """Launch shape of the learned Higgs update: 64 (default, also 0) or
+        32 sites per wave (fields do not depend on it)."""
+        check(self._lib.lceq_mlp_set_sites_per_wave(self.h, int(sites)))
+
     def eval(self, x):
         """The inference network (bf16 MFMA) on rows of the 10 raw inputs."""
         xx = np.ascontiguousarray(x, np.float32).reshape(-1, NIN)

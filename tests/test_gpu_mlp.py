@@ -85,10 +85,12 @@ def _bias_only_state(M):
     return st
 
 
+@pytest.mark.parametrize("spw", [32, 64])
 @pytest.mark.parametrize("shape", [(4, 4, 4), (6, 4, 8), (8, 8, 8), (10, 6, 8)])
-def test_bias_only_network_equals_oracle(lceq, oracle_lib, shape):
-    print("mlp bias-only vs oracle", shape, flush=True)
+def test_bias_only_network_equals_oracle(lceq, oracle_lib, shape, spw):
+    print("mlp bias-only vs oracle", shape, spw, flush=True)
     M = lceq.Mlp(seed=2)
+    M.set_sites_per_wave(spw)
     st = _bias_only_state(M)
     M.set_state(st)
     u, a = rand_conf(shape, 17, 0.4)
@@ -106,6 +108,35 @@ def test_bias_only_network_equals_oracle(lceq, oracle_lib, shape):
     og, oo = G.observables(), O.observables()
     for key in ("a02", "a04", "a0kin"):
         np.testing.assert_allclose(og[key], oo[key], rtol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (10, 6, 8)])
+def test_sites_per_wave_gives_the_same_chain(lceq, shape):
+    """32 and 64 sites per wave place a site in different MFMA columns and
+    lanes; each column is computed on its own, so the chains agree bit for
+    bit (untrained network: the proposal depends on a and F)."""
+    print("mlp sites per wave", shape, flush=True)
+    M = lceq.Mlp(seed=13)
+    u, a = rand_conf(shape, 23, 0.6)
+    runs = []
+    for spw in (32, 64):
+        M.set_sites_per_wave(spw)
+        G = lceq.Lattice(*shape)
+        G.set_couplings(*PROD)
+        G.put_links(u)
+        G.put_higgs(a)
+        G.rng_philox(91, 0)
+        acc = [G.higgs_mlp(M) for _ in range(3)]
+        G.sweep_mlp(M, 2, n_or=2, measflag=1)
+        runs.append((acc, G.get_higgs(), G.get_links(), G.history(2)))
+        G.close()
+    assert runs[0][0] == runs[1][0]
+    for i in (1, 2):
+        assert np.array_equal(runs[0][i], runs[1][i])
+    # the observables are sums over blocks, whose number differs
+    np.testing.assert_allclose(runs[0][3], runs[1][3], rtol=1e-12, atol=1e-15)
+    with pytest.raises(lceq.LceqError, match="sites per wave"):
+        M.set_sites_per_wave(48)
 
 
 def _single_site_moments(b2, b4):
