@@ -38,12 +38,12 @@ struct lceq_lattice {
   // device fields
   double *U, *A, *Uf, *Af, *Vst, *F;
   double *partials;  // [LCEQ_NSLOT][LCEQ_NQ][pstride]
-  int pstride;       // >= number of 256-site blocks
+  int pstride;       // >= number of 64-site blocks (the smallest launch shape)
   double *obs;       // [LCEQ_OBS_N]
   double *hist;      // [hist_cap][LCEQ_OBS_N] per-sweep records
   int hist_cap;
   int hist_rows;     // rows recorded since the last lceq_history
-  double *slices;    // correlator slice sums [14][lz]
+  double *slices;    // correlator slice partial sums [14][lz][corr_split]
   double *corr;      // correlator outputs [5][lz] + 4 scalars
   double *scratch;   // generic device scratch for host-array entry points
   size_t scratch_bytes;

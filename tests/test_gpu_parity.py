@@ -179,7 +179,10 @@ def test_gauge_fixing(make, omake, shape, alpha):
         assert_close(G.get_links_fixed(), O.get_links_fixed(), 1e-9, "u_f landau")
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 4, 8), (16, 16, 32)])
+# (2, 2, 600): the slice means do not fit the correlator kernel's LDS
+# (14 L_z doubles > 64 KB) and stay in device memory; (128, 96, 2): a slice
+# of 12,288 sites is split over the maximum of 16 blocks
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 4, 8), (16, 16, 32), (2, 2, 600), (128, 96, 2)])
 def test_correlators(make, omake, shape):
     G, O = _pair(make, omake, shape, 6)
     uf, af = rand_conf(shape, 60)
