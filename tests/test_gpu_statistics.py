@@ -104,3 +104,57 @@ def test_independent_chains_agree(oracle_lib):
         print("screening mass undefined: gpu %s cpu %s" % (mg, mo), flush=True)
     bad = {k: v for k, v in zs.items() if not abs(v) < 3.0}
     assert not bad, bad
+
+
+def test_learned_proposal_chain_agrees_with_plain():
+    """The learned Higgs update at interacting couplings (trace convention
+    beta_A = 0.36, broken phase): a trained network's chain and the
+    plain Metropolis chain, both on the GPU from one thermalised start with
+    independent Philox streams, give the same <plaquette>, <Tr A^2> and
+    <a.F> within errors (Madras-Sokal tau_int).  Exactness for any weights
+    is what this checks at beta_A != 0; the beta_A = 0 limit is checked
+    against quadrature in test_gpu_mlp.py."""
+    import lceqcd_amd
+    from lceqcd_amd.analysis import mean_error
+    shape = (16, 16, 16)
+    coup = lceqcd_amd.couplings_from_trace_convention(9.0, 0.36, -0.9, 0.1)
+    print("learned vs plain chain", shape, flush=True)
+    G = lceqcd_amd.Lattice(*shape)
+    G.set_couplings(*coup)
+    G.hotstart(55, 0.5)
+    G.rng_philox(55, 0)
+    d = G.sweep(0, 600, n_or=4, delta=0.4, measflag=0)
+    M = lceqcd_amd.Mlp(seed=5)
+    for _ in range(64):
+        d = G.sweep(0, 2, n_or=4, delta=d, measflag=0)
+        M.collect(G, nsteps=8, stride=1)
+    M.train(600, batch=4096, lr=2e-3, seed=5)
+    u0, a0 = G.get_links(), G.get_higgs()
+    rows = {}
+    for learned in (False, True):
+        X = lceqcd_amd.Lattice(*shape)
+        X.set_couplings(*coup)
+        X.put_links(u0)
+        X.put_higgs(a0)
+        X.rng_philox(600 + learned, 0)
+        h = []
+        for _ in range(4):
+            if learned:
+                X.sweep_mlp(M, 1000, n_or=4, measflag=1)
+            else:
+                d = X.sweep(0, 1000, n_or=4, delta=d, measflag=1)
+            h.append(X.history(1000))
+        rows[learned] = np.concatenate(h)[200:]
+        X.close()
+    bad = {}
+    for i, nm in ((0, "plaquette"), (2, "rho"), (1, "a.F/6")):
+        (m0, e0), (m1, e1) = mean_error(rows[False][:, i]), mean_error(rows[True][:, i])
+        z = (m1 - m0) / np.hypot(e0, e1)
+        print("%-10s plain %.6g +- %.2g  learned %.6g +- %.2g  z = %+.2f" % (nm, m0, e0, m1, e1, z),
+              flush=True)
+        if not abs(z) < 3.0:
+            bad[nm] = z
+    acc = rows[True][:, 8].mean()
+    print("learned acceptance %.3f plain %.3f" % (acc, rows[False][:, 8].mean()), flush=True)
+    assert acc > rows[False][:, 8].mean()
+    assert not bad, bad
