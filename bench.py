@@ -43,7 +43,9 @@ its shipped inputfile (beta = 9, x = 0.1, y = 0.5, imode 0): one step =
 ahiggs2 + ahiggs3 + agauge + renorm; value = site-updates/s of that sweep.
 Then Landau gauge fixing (lambda = 1.7, alpha = 1, limit_or = 1e-10) from the
 last configuration, timed per iteration, and the three z-correlator
-measurements; each dominant launch is timed with HIP events.
+measurements; each dominant launch is timed with HIP events.  cpu_baseline:
+the reference's own routines (oracle/_ref/refharness64_64, built from its
+sources with 8-byte reals) timed on the same configuration, when built.
 
     python bench.py --calibrate-copy
 
@@ -181,6 +183,27 @@ GFIX_PASS_BYTES_PER_SITE = 216   # all links read + written, a_f of one parity r
 SLICES_BYTES_PER_SITE = 144      # u_f 96 + a_f 24 + a 24
 
 
+def cpu_baseline_reference(L, V, seconds):
+    """The reference's own CPU routines (oracle/_ref: its ahiggs2, ahiggs3,
+    agauge and renorm built with 8-byte reals, one rank, array capacity
+    64^3) on the lattice's current configuration: about `seconds` of sweeps.
+    None when that build is absent."""
+    from oracle import refrun
+    if not os.access(refrun.big_harness_path(64), os.X_OK):
+        return None
+    u, a = L.get_links(), L.get_higgs()
+    from lceqcd_amd import couplings_from_xy
+    ba, b2, b4 = couplings_from_xy(9.0, 0.1, 0.5)
+    kw = dict(prec=64, beta=9.0, beta_a=ba, beta_2=b2, beta_4=b4, imode=0)
+    t1, _ = refrun.time_sweeps(L.shape, u, a, 1, **kw)
+    k = max(1, min(100, int(seconds / max(t1, 1e-3))))
+    tk, _ = refrun.time_sweeps(L.shape, u, a, k, **kw)
+    return {"value": V * k / tk, "unit": "site-updates/s", "cores": 1, "kind": "reference",
+            "sample": "%d sweeps of f/main.f:37-41 (the reference's ahiggs2, ahiggs3, agauge, "
+                      "renorm; 8-byte reals, one rank) on the same %dx%dx%d configuration, "
+                      "%.1f s timed inside oracle/_ref/refharness64_64" % ((k,) + L.shape + (tk,))}
+
+
 def refsched64(args):
     """The reference schedule, gauge fixing and correlators at 64^3."""
     import lceqcd_amd
@@ -278,6 +301,10 @@ def refsched64(args):
                                    "a04": hist[-1, 3], "a0kin": hist[-1, 1],
                                    "link_accept": hist[-1, 7], "higgs_accept": hist[-1, 8]},
     }
+    if not args.no_cpu:
+        cb = cpu_baseline_reference(L, V, args.cpu_seconds)
+        if cb is not None:
+            rec["cpu_baseline"] = cb
     print(json.dumps(rec), flush=True)
 
 

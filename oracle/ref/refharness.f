@@ -12,6 +12,10 @@
 *     the captured stream.  The extra draw after the run must equal the
 *     captured extra uniform; that proves the consumption count.
 *
+*     Command 16 times nrep sweeps of the reference's production schedule
+*     (f/main.f:37-41) with the compiler's system_clock, without the
+*     uniform capture, and writes the elapsed seconds and the last plaq0.
+*
 *     Commands 14 (confsave) and 15 (oldconf) run the reference's
 *     configuration I/O in the working directory on a 1x1x1 Cartesian
 *     communicator: 14 writes the confsave files for beta, x = lambda,
@@ -43,6 +47,7 @@
       complex, allocatable :: vv(:,:)
       integer lx,ly,lz,cmd,arg1,arg2,nrep,iseedv,n,i,j,eo,mu,k,rep
       integer ns, nacch, iter1, nuni
+      integer(8) clk0, clk1, clkrate
       integer, allocatable :: sd(:)
       real acc, delta, da1, rnext
       character(len=512) fin, fout
@@ -295,6 +300,18 @@
           call wru(11, n)
           call wra(11, n)
         endif
+      else if (cmd .eq. 16) then
+        call system_clock(clk0, clkrate)
+        do rep = 1, nrep
+          call ahiggs2(1, acc)
+          call ahiggs3(1, acc)
+          call agauge(1, acc)
+          call renorm
+        enddo
+        call system_clock(clk1)
+        rnext = real(clk1 - clk0) / real(clkrate)
+        call wrs(11, rnext)
+        call wrs(11, plaq0)
       else if (cmd .eq. 13) then
         allocate(rr(n), xx(n), outb(n))
         do i = 1, n

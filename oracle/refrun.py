@@ -16,7 +16,7 @@ REF_DIR = os.path.join(_HERE, "_ref")
 
 CMD = dict(geom=1, staples=2, agauge=3, a_force=4, ahiggs1=5, ahiggs2=6,
            ahiggs3=7, renorm=8, or_gauge_fixing=9, landau_gauge=10, sweep=11,
-           heat_kpi=12, gauss=13, confsave=14, oldconf=15)
+           heat_kpi=12, gauss=13, confsave=14, oldconf=15, time=16)
 
 
 def harness_path(prec=64):
@@ -25,6 +25,30 @@ def harness_path(prec=64):
 
 def available(prec=64):
     return os.access(harness_path(prec), os.X_OK)
+
+
+def _unlimited_stack():
+    import resource
+    hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def big_harness_path(prec=64):
+    """The harness built with 64^3 array capacity (make -C oracle/ref big)."""
+    return os.path.join(REF_DIR, "refharness%d_64" % prec)
+
+
+def time_sweeps(shape, u, a, nsweep, *, prec=64, seed=314159, beta=9.0, beta_a=0.0,
+                beta_2=1.0, beta_4=0.0, imode=0):
+    """Seconds the reference's own routines take for nsweep sweeps of
+    f/main.f:37-41 (ahiggs2, ahiggs3, agauge, renorm) from (u, a), timed
+    inside the harness (command 16, no uniform capture); also the last
+    plaq0.  Lattices above 24^3 need the 64^3-capacity build."""
+    big = max(shape) > 24
+    exe = big_harness_path(prec) if big else harness_path(prec)
+    recs = run("time", shape, u, a, nrep=nsweep, seed=seed, beta=beta, beta_a=beta_a,
+               beta_2=beta_2, beta_4=beta_4, imode=imode, prec=prec, exe=exe)
+    return float(recs[0][0]), float(recs[1][0])
 
 
 DROPIN_HARNESS = os.path.join(REF_DIR, "dropin_harness64")
@@ -62,8 +86,11 @@ def run(cmd, shape, u, a, *, arg1=0, arg2=0, imode=0, nrep=1, seed=12345,
             f.write(ex2.tobytes())
         e = dict(os.environ)
         e.update(env or {})
+        # the reference keeps large automatic arrays on the stack
+        # (f/agauge.f:36-38), as its run scripts allow with an unlimited stack
         r = subprocess.run([exe or harness_path(prec), fin, fout], env=e, timeout=300,
-                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, cwd=workdir)
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, cwd=workdir,
+                           preexec_fn=_unlimited_stack)
         if r.returncode != 0:
             raise RuntimeError("harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:],
                                                               r.stderr[-2000:]))
