@@ -36,6 +36,7 @@ extern "C" {
 #define LCEQ_ERR_RNG (-3)         /* fed uniform stream exhausted */
 #define LCEQ_ERR_NOT_CONVERGED (-4) /* landau_gauge hit max_iter */
 #define LCEQ_ERR_IO (-5)          /* configuration file missing, short or unwritable */
+#define LCEQ_ERR_FIELD (-6)       /* lceq_check_fields: non-finite or non-unitary fields */
 
 /* uniform sources */
 #define LCEQ_RNG_FED 0
@@ -145,6 +146,12 @@ int lceq_ahiggs2(lceq_lattice *L, int measflag, double *accept);
 int lceq_ahiggs3(lceq_lattice *L, int measflag, double *accept);
 /* renorm(), f/renorm.f:6 (called at f/main.f:24,41) */
 int lceq_renorm(lceq_lattice *L);
+/* Field guard (no reference counterpart; the reference has no checks and
+ * aborts only on MPI errors): max_dev = max over links of
+ * | |alpha|^2 + |beta|^2 - 1 | (finite links), nonfinite = number of NaN or
+ * infinite link and Higgs values.  Returns LCEQ_ERR_FIELD when nonfinite > 0
+ * or, for tol >= 0, when max_dev > tol.  Either output may be NULL. */
+int lceq_check_fields(lceq_lattice *L, double tol, double *max_dev, int64_t *nonfinite);
 /* u_f_renorm(), f/u_f_renorm.f:6 (called at f/landau_gauge.f:57) */
 int lceq_u_f_renorm(lceq_lattice *L);
 /* or_gauge_fixing(da1), f/or_gauge_fixing1.f:5 (called at f/landau_gauge.f:56) */

@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -586,6 +587,31 @@ extern "C" int lceq_renorm(lceq_lattice* L) {
   launch_renorm(L, L->U);
   return check_launch();
 }
+extern "C" int lceq_check_fields(lceq_lattice* L, double tol, double* max_dev,
+                                 int64_t* nonfinite) {
+  NEEDL(L);
+  int nb = blocks_for(L->g.n);
+  int rc = ensure_scratch(L, (size_t)2 * nb * sizeof(double));
+  if (rc) return rc;
+  launch_check_fields(L, L->scratch);
+  if ((rc = check_launch())) return rc;
+  std::vector<double> h(2 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(h.data(), L->scratch, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  double dev = 0.0, bad = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    dev = std::max(dev, h[2 * b]);
+    bad += h[2 * b + 1];
+  }
+  if (max_dev) *max_dev = dev;
+  if (nonfinite) *nonfinite = (int64_t)bad;
+  if (bad > 0.0) return fail(LCEQ_ERR_FIELD, "%.0f non-finite link or Higgs values", bad);
+  if (tol >= 0.0 && dev > tol)
+    return fail(LCEQ_ERR_FIELD, "links off SU(2) by %.3g (tolerance %.3g)", dev, tol);
+  return LCEQ_OK;
+}
+
 extern "C" int lceq_u_f_renorm(lceq_lattice* L) {
   NEEDL(L);
   launch_renorm(L, L->Uf);

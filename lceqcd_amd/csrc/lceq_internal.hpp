@@ -113,6 +113,8 @@ void launch_force(lceq_lattice* L, int eo);
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
                   uint64_t pass);
 void launch_renorm(lceq_lattice* L, double* U);
+// per-block (unitarity deviation, non-finite count) of U and A into part[2 nb]
+void launch_check_fields(lceq_lattice* L, double* part);
 // One half-iteration of gauge fixing; check = 1 first tests the loop
 // condition for the `iter` iterations already complete (see k_gfix).
 void launch_gfix(lceq_lattice* L, int eo, int compute_da, int renorm, int guarded, int check = 0,

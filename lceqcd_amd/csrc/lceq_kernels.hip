@@ -315,6 +315,54 @@ void launch_renorm(lceq_lattice* L, double* U) {
   hipLaunchKernelGGL(k_renorm, dim3(blocks_for(L->g.n), 6), dim3(TB), 0, L->stream, L->g, U);
 }
 
+// Field guard: per block of sites (both parities), the largest unitarity
+// deviation | |alpha|^2 + |beta|^2 - 1 | of the finite links and the number
+// of non-finite values among the links and the Higgs field:
+// part[2 blk] = deviation, part[2 blk + 1] = count.
+__global__ __launch_bounds__(256) void k_check_fields(Geom g, const double* __restrict__ U,
+                                                      const double* __restrict__ A,
+                                                      double* __restrict__ part) {
+  __shared__ double wmax[4], wbad[4];
+  int s = blockIdx.x * TB + threadIdx.x;
+  double dev = 0.0, bad = 0.0;
+  if (s < g.n) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+      for (int mu = 0; mu < 3; ++mu) {
+        su2 u = ld_link(U, g, mu, p, s);
+        double c[4] = {u.a.re, u.a.im, u.b.re, u.b.im};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) bad += isfinite(c[k]) ? 0.0 : 1.0;
+        double d = fabs(((c[0] * c[0] + c[1] * c[1]) + (c[2] * c[2] + c[3] * c[3])) - 1.0);
+        if (isfinite(d)) dev = fmax(dev, d);
+      }
+      vec3 a = ld_higgs(A, g, p, s);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) bad += isfinite(a.x[k]) ? 0.0 : 1.0;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    dev = fmax(dev, __shfl_xor(dev, off, 64));
+    bad += __shfl_xor(bad, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wmax[threadIdx.x >> 6] = dev;
+    wbad[threadIdx.x >> 6] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    part[2 * blockIdx.x + 1] = ((wbad[0] + wbad[1]) + wbad[2]) + wbad[3];
+  }
+}
+
+void launch_check_fields(lceq_lattice* L, double* part) {
+  hipLaunchKernelGGL(k_check_fields, dim3(blocks_for(L->g.n)), dim3(TB), 0, L->stream, L->g,
+                     L->U, L->A, part);
+}
+
 // ========================================================= gauge fixing
 
 struct GfArgs {

@@ -223,6 +223,16 @@ module lceq_c
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    ! field guard (no reference counterpart): LCEQ_ERR_FIELD on non-finite
+    ! values or, for tol >= 0, links further than tol from SU(2)
+    integer(c_int) function lceq_check_fields(h, tol, max_dev, nonfinite) &
+        bind(C, name="lceq_check_fields")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: max_dev
+      integer(c_int64_t), intent(out) :: nonfinite
+    end function
     ! u_f_renorm(), f/u_f_renorm.f:6
     integer(c_int) function lceq_u_f_renorm(h) bind(C, name="lceq_u_f_renorm")
       import :: c_int, c_ptr

@@ -37,11 +37,11 @@ program lceq_main
   character(len=256) :: fin, arg
   character(len=49) :: action_file, joblog_file, corr_file, checkp_file, afield_file
   integer :: betaint, xint, yint, isweep, meas, mstart, nmeas, iter1, iter_or, z, ntherm
-  integer(c_int64_t) :: seed
+  integer(c_int64_t) :: seed, nbad
   integer :: realbytes
   character(len=49) :: u_conf_file, a_conf_file
   type(c_ptr) :: h
-  real(8) :: acc1, acc2, acc3, acco, acch, accg, da1, da, obs(7)
+  real(8) :: acc1, acc2, acc3, acco, acch, accg, da1, da, obs(7), maxdev
   real(8), allocatable :: corr_a(:), corr_u(:), corr_3(:)
   integer :: u_act = 3, u_job = 4, u_cor = 31, u_afi = 32, u_chk = 33
 
@@ -172,6 +172,9 @@ program lceq_main
     write(u_act, '(i5,1x,4(e23.16,1x))') isweep + confstart, obs(1), obs(2), obs(3), obs(4)
     if (mod(isweep, imeas) == 0) then
       meas = meas + 1
+      ! guard: abort, as the reference does on errors, if the fields have
+      ! gone non-finite or the links have left SU(2)
+      call lceq_check(lceq_check_fields(h, 1d-10, maxdev, nbad), 'check_fields')
       call lceq_check(lceq_landau_gauge(h, da1, iter1), 'landau_gauge')
       da = da + da1
       iter_or = iter_or + iter1

@@ -256,6 +256,15 @@ class Lattice:
         """f/renorm.f:6"""
         check(self._L.lceq_renorm(self.h))
 
+    def check_fields(self, tol=-1.0):
+        """Field guard: (max unitarity deviation of the links, number of
+        non-finite link and Higgs values); raises LceqError if any value is
+        non-finite or, for tol >= 0, the deviation exceeds tol."""
+        dev = C.c_double()
+        bad = C.c_int64()
+        check(self._L.lceq_check_fields(self.h, float(tol), C.byref(dev), C.byref(bad)))
+        return dev.value, bad.value
+
     def u_f_renorm(self):
         """f/u_f_renorm.f:6"""
         check(self._L.lceq_u_f_renorm(self.h))
