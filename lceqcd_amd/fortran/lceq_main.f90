@@ -9,11 +9,14 @@
 ! reference's names (f/measinit.f:246-273) and record formats (f/meas1.f:91,
 ! f/correlator.f:168-181, f/meas2a.f:83-97, f/meas1.f:70-76).
 !
-!   lceq_main [inputfile] [seed] [ntherm] [realbytes]
+!   lceq_main [inputfile] [seed] [ntherm] [realbytes] [corrflag]
 !
 ! ntherm (default 100, as f/main.f:20) is the number of thermalisation sweeps;
 ! realbytes (8, or 4 for files of the as-shipped single-precision build) is
-! the real size of the configuration files.
+! the real size of the configuration files.  corrflag selects the
+! measurement of f/main.f:55-79 (the reference fixes corr_flag = 1 in its
+! source): 1 Landau gauge + the correlators (correl_, checkp_, afield_
+! files), 2 polyakov + scalarcorr (polyak_, corra2_ files), 3 both.
 !
 ! Differences from the reference, by design: one device holds the whole
 ! lattice (dimproc and nproc are read and ignored); the uniforms come from the
@@ -36,14 +39,17 @@ program lceq_main
   real(8), parameter :: sigma = 3.1759114d0
   character(len=256) :: fin, arg
   character(len=49) :: action_file, joblog_file, corr_file, checkp_file, afield_file
+  character(len=49) :: polyak_file, corra2_file
   integer :: betaint, xint, yint, isweep, meas, mstart, nmeas, iter1, iter_or, z, ntherm
   integer(c_int64_t) :: seed, nbad
-  integer :: realbytes
+  integer :: realbytes, corr_flag
+  real(8) :: pmean
+  real(8), allocatable :: pcorr(:)
   character(len=49) :: u_conf_file, a_conf_file
   type(c_ptr) :: h
   real(8) :: acc1, acc2, acc3, acco, acch, accg, da1, da, obs(7), maxdev
   real(8), allocatable :: corr_a(:), corr_u(:), corr_3(:)
-  integer :: u_act = 3, u_job = 4, u_cor = 31, u_afi = 32, u_chk = 33
+  integer :: u_act = 3, u_job = 4, u_cor = 31, u_afi = 32, u_chk = 33, u_pol = 34, u_ca2 = 35
 
   fin = 'inputfile'
   if (command_argument_count() >= 1) call get_command_argument(1, fin)
@@ -61,6 +67,15 @@ program lceq_main
   if (command_argument_count() >= 4) then
     call get_command_argument(4, arg)
     read(arg, *) realbytes
+  end if
+  corr_flag = 1
+  if (command_argument_count() >= 5) then
+    call get_command_argument(5, arg)
+    read(arg, *) corr_flag
+  end if
+  if (corr_flag < 1 .or. corr_flag > 3) then
+    write(*, *) 'corrflag must be 1, 2 or 3:', corr_flag
+    call abort()
   end if
 
   ! ---- f/paraminit.f:82-101
@@ -134,12 +149,16 @@ program lceq_main
   write(corr_file, 1000) 'correl', nl(3), betaint, xint, yint, confstart
   write(afield_file, 1000) 'afield', nl(3), betaint, xint, yint, confstart
   write(checkp_file, 1000) 'checkp', nl(3), betaint, xint, yint, confstart
+  write(polyak_file, 1000) 'polyak', nl(3), betaint, xint, yint, confstart
+  write(corra2_file, 1000) 'corra2', nl(3), betaint, xint, yint, confstart
   call header(u_act, action_file)
   call header(u_job, joblog_file)
   call header(u_cor, corr_file)
   call header(u_afi, afield_file)
   call header(u_chk, checkp_file)
-  allocate(corr_a(nl(3)), corr_u(nl(3)), corr_3(nl(3)))
+  call header(u_pol, polyak_file)
+  call header(u_ca2, corra2_file)
+  allocate(corr_a(nl(3)), corr_u(nl(3)), corr_3(nl(3)), pcorr(nl(3)))
 
   ! ---- thermalisation, f/main.f:20-25
   do isweep = 1, ntherm
@@ -175,6 +194,11 @@ program lceq_main
       ! guard: abort, as the reference does on errors, if the fields have
       ! gone non-finite or the links have left SU(2)
       call lceq_check(lceq_check_fields(h, 1d-10, maxdev, nbad), 'check_fields')
+      if (corr_flag == 2) then  ! f/main.f:68-71
+        call polyakov_out()
+        call scalarcorr_out()
+        cycle
+      end if
       call lceq_check(lceq_landau_gauge(h, da1, iter1), 'landau_gauge')
       da = da + da1
       iter_or = iter_or + iter1
@@ -197,6 +221,7 @@ program lceq_main
       open(u_afi, file=afield_file, status='old', position='append')
       write(u_afi, '(i5.5,1x,3(e23.16,1x))') meas + mstart, obs(5), obs(6), obs(7)
       close(u_afi)
+      if (corr_flag == 3) call polyakov_out()  ! f/main.f:73-78
       if (meas == nmeas) then
         open(u_job, file=joblog_file, status='old', position='append')
         write(u_job, '(a,1x,i20)') 'number of iteration for or. gauge fixing :', iter_or / nmeas
@@ -222,6 +247,30 @@ program lceq_main
 1000 format(a6, '_s', i2.2, '_be', i5.5, '_x', i5.5, '_y', i5.5, '_u', i6.6)
 
 contains
+
+  ! f/polyakov.f:110-117: record number, mean(P)^2, the correlator
+  subroutine polyakov_out()
+    call lceq_check(lceq_polyakov(h, pmean, pcorr), 'polyakov')
+    open(u_pol, file=polyak_file, status='old', position='append')
+    write(u_pol, '(i5.5)') mstart + meas
+    write(u_pol, '(e23.16)') pmean
+    do z = 1, nl(3)
+      write(u_pol, '(e23.16)') pcorr(z)
+    end do
+    close(u_pol)
+  end subroutine
+
+  ! f/scalarcorr.f:106-113: record number, mean(rho), the correlator
+  subroutine scalarcorr_out()
+    call lceq_check(lceq_scalarcorr(h, pmean, pcorr), 'scalarcorr')
+    open(u_ca2, file=corra2_file, status='old', position='append')
+    write(u_ca2, '(i5.5)') mstart + meas
+    write(u_ca2, '(e23.16)') pmean
+    do z = 1, nl(3)
+      write(u_ca2, '(e23.16)') pcorr(z)
+    end do
+    close(u_ca2)
+  end subroutine
 
   ! configuration file names: f/confsave.f:249-253 (saved) and
   ! f/measinit.f:246-250 (the case oldconf opens)

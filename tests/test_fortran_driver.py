@@ -111,3 +111,48 @@ def test_driver_restarts_from_saved_configuration(tmp_path):
     u2, a2 = conf_read((8, 8, 8), tmp_path / "u_conf_s08_BE09000_x01000_y05000_U000000",
                        tmp_path / "a_conf_s08_BE09000_x01000_y05000_U000000")
     assert np.array_equal(u2, u) and np.array_equal(a2, a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("corrflag", [2, 3])
+def test_driver_polyakov_and_scalarcorr_outputs(tmp_path, oracle_lib, corrflag):
+    """corrflag 2 (polyakov + scalarcorr, f/main.f:68-71) and 3 (gauge fixing,
+    the correlators and polyakov, f/main.f:73-78): the polyak_ and corra2_
+    files carry the reference's records (f/polyakov.f:110-117,
+    f/scalarcorr.f:106-113) and their values equal the CPU restatement's run
+    of the same schedule and stream."""
+    exe = os.path.join(FDIR, "lceq_main")
+    (tmp_path / "inputfile").write_text(INPUTFILE.format(init=0, nsweep="00002"))
+    print("running lceq_main 8^3 nsweep=2 ntherm=0 corrflag", corrflag, flush=True)
+    r = subprocess.run([exe, "inputfile", "4242", "0", "8", str(corrflag)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    def records(prefix):
+        lines = open(glob.glob(str(tmp_path / (prefix + "_s08_*")))[0]).read().splitlines()
+        body = lines[3:]
+        return [(int(body[k * 10]), [float(v) for v in body[k * 10 + 1:k * 10 + 10]])
+                for k in range(len(body) // 10)]
+
+    pol = records("polyak")
+    ca2 = records("corra2")
+    assert [p[0] for p in pol] == [1, 2]
+    assert len(ca2) == (2 if corrflag == 2 else 0)
+    from lceqcd_amd import couplings_from_xy
+    O = oracle_lib.OracleLattice(8, 8, 8)
+    O.set_couplings(9.0, *couplings_from_xy(9.0, 0.1, 0.5))
+    O.hotstart(4242, 0.5)
+    O.philox(4242, 0)
+    for it in range(2):
+        O.ahiggs2(1)
+        O.ahiggs3(1)
+        O.agauge(1)
+        O.renorm()
+        if corrflag == 3:
+            O.set_gauge_fixing(1.7, 1.0, 1e-10)
+            O.landau_gauge()
+        m, c = O.polyakov()
+        np.testing.assert_allclose(pol[it][1], [m] + list(c), rtol=1e-12, atol=1e-14)
+        if corrflag == 2:
+            m, c = O.scalarcorr()
+            np.testing.assert_allclose(ca2[it][1], [m] + list(c), rtol=1e-12, atol=1e-14)
