@@ -5,7 +5,7 @@ tau_int = 1/2."""
 import numpy as np
 import pytest
 
-from lceqcd_amd.analysis import autocorrelation, mean_error, tau_int
+from lceqcd_amd.analysis import autocorrelation, cosh_mass, mean_error, screening_mass, tau_int
 
 
 def ar1(phi, n, seed):
@@ -38,3 +38,28 @@ def test_mean_error_covers_the_truth():
         m, e = mean_error(ar1(0.9, 20000, 100 + s))
         hits += abs(m) < 2 * e
     assert hits >= 32  # ~95% coverage expected
+
+
+def _cosh_corr(L, m, A=2.0, B=5.0):
+    z = np.arange(L)
+    return A * np.cosh(m * (z - L / 2)) + B
+
+
+@pytest.mark.parametrize("L,m", [(16, 0.7), (32, 0.25), (64, 1.3)])
+def test_cosh_mass_exact(L, m):
+    c = _cosh_corr(L, m)
+    for z in range(1, L // 2 - 2):
+        assert abs(cosh_mass(c, z) - m) < 1e-9 * max(1.0, m)
+    assert np.isnan(cosh_mass(c, 0))
+
+
+def test_screening_mass_jackknife_covers_the_truth():
+    rng = np.random.default_rng(4)
+    hits = 0
+    for s in range(40):
+        c = _cosh_corr(16, 0.7)
+        samples = c + rng.normal(scale=0.05, size=(400, 16))
+        m, e = screening_mass(samples, z=1, nblk=20)
+        assert e > 0
+        hits += abs(m - 0.7) < 2 * e
+    assert hits >= 32
