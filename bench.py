@@ -70,10 +70,11 @@ HBM_PEAK_GBPS = 8000.0
 LINK_PASS_BYTES_PER_SITE = 136
 HIGGS_PASS_BYTES_PER_SITE = 132
 SWEEP_BYTES_PER_SITE = 6 * LINK_PASS_BYTES_PER_SITE * 5 + 2 * HIGGS_PASS_BYTES_PER_SITE
-# beyond-L2 bytes of one 64^3 overrelaxation launch, FETCH_SIZE x 2.00 +
-# WRITE_SIZE x 1.00 (factors from a 512 MiB device copy), measured with
-# rocprofv3 --pmc (profiles/k_link_or_64cubed_hbm_traffic.md)
-OR_TRAFFIC_64 = 33.74e6 + 4.04e6
+# beyond-L2 bytes of one 64^3 overrelaxation launch by threads per block
+# (XCD-aware order), FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
+# 512 MiB device copy), measured with rocprofv3 --pmc
+# (profiles/k_link_or_64cubed_hbm_traffic.md)
+OR_TRAFFIC_64 = {512: 33.69e6 + 4.03e6, 256: 33.73e6 + 4.04e6}
 
 
 def couplings(beta_a_trace=0.33):
@@ -515,7 +516,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": OR_TRAFFIC_64 if shape == (64, 64, 64) else None,
+            "traffic": (OR_TRAFFIC_64.get(L.get_shape()[0]) if shape == (64, 64, 64)
+                        and L.get_shape()[1] else None),
             "bytes_per_launch": launch_bytes,
             "avg_launch_us": avg_s * 1e6,
             "launches": or_n,

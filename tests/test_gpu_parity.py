@@ -3,9 +3,10 @@ vectors and against the CPU restatement, on the same inputs.
 
 Bar: 1e-12 relative in FP64 for fields and observables (tolerances are stated
 per case in tests/parity_cases.py), exact for counts, zero accept/reject flips
-(a flip would show up as an O(1) field difference and fail the bar); bit for
-bit for the updates whose fields involve no transcendental function (link
-overrelaxation, ahiggs1, ahiggs3)."""
+(a flip shows up as an O(1) field difference at a site and is counted).  The
+library fuses multiply-adds where the compiler can; the CPU restatement rounds
+every operation as the reference's x86 build does, so fields differ by a few
+ulp per update."""
 import numpy as np
 import pytest
 
@@ -70,30 +71,47 @@ def test_philox_link_updates(make, omake, shape):
     assert G.rng_state()[1] == O.rng_state()[1]
 
 
+def _compare_sites(got, exp, what):
+    """Largest difference, and the number of sites (last axis = components)
+    whose update went the other way: an accept/reject flip moves a site by
+    the size of a proposal, a rounding difference by a few ulp."""
+    d = np.abs(np.asarray(got) - np.asarray(exp)).max(axis=-1)
+    flips = int((d > 1e-9).sum())
+    print("%s: max deviation %.2e, flips %d" % (what, d.max(), flips), flush=True)
+    return d.max(), flips
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
-def test_philox_or_and_metropolis_bitwise(make, omake, shape):
-    """Link overrelaxation, ahiggs1 and ahiggs3 use exp only to decide an
-    accept, and the library is built without floating-point contraction, so
-    their fields equal the oracle's to the bit (only an exp result differing
-    by an ulp exactly at the accept threshold could break this)."""
-    print("bitwise OR/Metropolis", shape, flush=True)
+def test_philox_or_and_metropolis_flips_counted(make, omake, shape):
+    """Link overrelaxation, ahiggs1 and ahiggs3 (no transcendental function
+    in the fields; exp only decides the accept), each update on identical
+    inputs (the CPU restatement takes the GPU's fields before every update;
+    the Philox pass counters advance together): the GPU (fused multiply-adds)
+    and the restatement (every operation rounded as written) agree within
+    1e-12 and no accept/reject decision differs."""
+    print("OR/Metropolis flips", shape, flush=True)
     G, O = _pair(make, omake, shape, 4, coup=PROD, asig=1.0)
-    for k in range(2):
-        for L in (G, O):
-            L.link_or(1)
-        assert np.array_equal(G.get_links().view(np.int64), O.get_links().view(np.int64))
+
+    def sync():
+        O.put_links(G.get_links())
+        O.put_higgs(G.get_higgs())
+
     dG = dO = 0.4
-    for k in range(2):
-        dG, _ = G.ahiggs1(dG, 1)
-        dO, _ = O.ahiggs1(dO, 1)
-        assert dG == dO
-        assert np.array_equal(G.get_higgs().view(np.int64), O.get_higgs().view(np.int64))
-    G.ahiggs3(1)
-    O.ahiggs3(1)
-    assert np.array_equal(G.get_higgs().view(np.int64), O.get_higgs().view(np.int64))
-    G.link_or(1)
-    O.link_or(1)
-    assert np.array_equal(G.get_links().view(np.int64), O.get_links().view(np.int64))
+    for k in range(3):
+        sync()
+        assert G.link_or(1) == O.link_or(1)
+        dev, flips = _compare_sites(G.get_links(), O.get_links(), "link OR %d" % k)
+        assert flips == 0 and dev < 1e-12
+        sync()
+        dG, aG = G.ahiggs1(dG, 1)
+        dO, aO = O.ahiggs1(dO, 1)
+        assert aG == aO and dG == dO  # equal accept counts, equal adapted step
+        dev, flips = _compare_sites(G.get_higgs(), O.get_higgs(), "ahiggs1 %d" % k)
+        assert flips == 0 and dev < 1e-12
+        sync()
+        assert G.ahiggs3(1) == O.ahiggs3(1)
+        dev, flips = _compare_sites(G.get_higgs(), O.get_higgs(), "ahiggs3 %d" % k)
+        assert flips == 0 and dev < 1e-12
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
