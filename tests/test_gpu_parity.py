@@ -136,12 +136,15 @@ def test_philox_higgs_updates(make, omake, shape, imode):
     assert_close(G.get_higgs(), O.get_higgs(), 1e-12, "higgs")
 
 
-@pytest.mark.parametrize("nsweep,tol", [(1, 1e-12), (2, 1e-12), (6, 1e-9)])
+@pytest.mark.parametrize("nsweep,tol", [(1, 1e-12), (2, 1e-11), (6, 1e-9)])
 def test_philox_production_sweeps_16(make, omake, nsweep, tol):
     """1 KP + 4 OR + ahiggs1 sweeps at the production couplings, GPU ==
-    oracle.  Roundoff differences (FMA contraction, libm ulps) grow by roughly
-    an order of magnitude per sweep in this chaotic Markov chain, so the
-    tolerance after six sweeps is wider."""
+    oracle as chains (each update on the other engine's output, not on
+    identical inputs).  Roundoff differences (fused multiply-adds, libm ulps)
+    grow by roughly an order of magnitude per sweep in this chaotic Markov
+    chain from a hot start, so the tolerance widens with the sweep count; the
+    1e-12 bar for one update on identical inputs is checked in
+    test_philox_link_updates and test_philox_or_and_metropolis_flips_counted."""
     shape = (16, 16, 16)
     G, O = _pair(make, omake, shape, 3, coup=PROD, asig=2.0)
     O.set_renorm_on_write(1)
