@@ -89,9 +89,11 @@ __device__ __forceinline__ double hop_action(double beta_a, const vec3& ax, cons
   return beta_a * m_nn(m_an(ax, u), m_ad(axp, u)).a.re;  // f/agauge.f:126-159
 }
 
+// U / |U| (f/renorm.f:24-34) with the reciprocal square root (within an ulp
+// of 1/sqrt, one operation instead of a square root and a division)
 __device__ __forceinline__ su2 normalized(su2 u) {
   double t = (u.a.re * u.a.re + u.a.im * u.a.im) + (u.b.re * u.b.re + u.b.im * u.b.im);
-  t = 1.0 / sqrt(t);
+  t = rsqrt(t);
   return su2{c_scale(t, u.a), c_scale(t, u.b)};
 }
 
@@ -180,7 +182,7 @@ __device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const
   su2 V = staple_sum(o.k);
   // f/agauge.f:71-84
   double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
-  t = 1.0 / sqrt(t);
+  t = rsqrt(t);
   su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
   su2 h;
   if (KP) {
