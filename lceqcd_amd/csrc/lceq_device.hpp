@@ -49,15 +49,21 @@ __device__ __forceinline__ su2 m_dd(su2 p, su2 q) {  // U1^+ U2^+ (matmul.f:118-
 struct vec3 {
   double x[3];
 };
+// (i y) z for real y: the product with x1 = (0, y) without its zero terms
+// (0 * z.re is +-0 and adding it changes no finite value, so the result is
+// the full complex product's, up to the sign of a zero)
+__device__ __forceinline__ cpx c_mul_i(double y, cpx z) { return cpx{-(y * z.im), y * z.re}; }
 __device__ __forceinline__ su2 m_ad(const vec3& a, su2 u) {  // X(a) U^+ (matmul.f:150-154)
-  cpx x1 = c_mk(0.0, a.x[2]), x2 = c_mk(a.x[1], a.x[0]);
-  return su2{c_add(c_mul(x1, c_conj(u.a)), c_mul(x2, c_conj(u.b))),
-             c_add(c_neg(c_mul(x1, u.b)), c_mul(x2, u.a))};
+  double y = a.x[2];
+  cpx x2 = c_mk(a.x[1], a.x[0]);
+  return su2{c_add(c_mul_i(y, c_conj(u.a)), c_mul(x2, c_conj(u.b))),
+             c_add(c_neg(c_mul_i(y, u.b)), c_mul(x2, u.a))};
 }
 __device__ __forceinline__ su2 m_an(const vec3& a, su2 u) {  // X(a) U (matmul.f:183-187)
-  cpx x1 = c_mk(0.0, a.x[2]), x2 = c_mk(a.x[1], a.x[0]);
-  return su2{c_sub(c_mul(x1, u.a), c_mul(x2, c_conj(u.b))),
-             c_add(c_mul(x1, u.b), c_mul(x2, c_conj(u.a)))};
+  double y = a.x[2];
+  cpx x2 = c_mk(a.x[1], a.x[0]);
+  return su2{c_sub(c_mul_i(y, u.a), c_mul(x2, c_conj(u.b))),
+             c_add(c_mul_i(y, u.b), c_mul(x2, c_conj(u.a)))};
 }
 
 // ------------------------------------------------------------ geometry
