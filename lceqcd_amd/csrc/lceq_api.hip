@@ -56,17 +56,13 @@ extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
     return fail(LCEQ_ERR_ARG, "extents must be even and >= 2 (got %d %d %d)", lx, ly, lz);
   if ((int64_t)lx * ly * lz * 3 > INT32_MAX)
     return fail(LCEQ_ERR_ARG, "lattice too large for 32-bit link counters");
+  if (ly >= (1 << 22) || lz >= (1 << 22))  // div_small's quotient range
+    return fail(LCEQ_ERR_ARG, "ly and lz must be below 2^22");
   lceq_lattice* L = new lceq_lattice();
   memset(L, 0, sizeof(*L));
+  L->g = geom_init(lx, ly, lz);
   Geom& g = L->g;
-  g.lx = lx;
-  g.ly = ly;
-  g.lz = lz;
-  g.h1 = lx / 2;
-  g.h2 = lx * ly / 2;
   L->vol = lx * ly * lz;
-  g.n = L->vol / 2;
-  g.npad = (g.n + 63) & ~63;
   L->tb = 256;
   L->xcd = 1;
   L->pstride = (g.n + 63) / 64 + 8;  // block partials of the smallest launch shape

@@ -278,15 +278,13 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
       delete B;
       return bfail(LCEQ_ERR_ARG, "replica %d: extents must be even and >= 2", r);
     }
+    if ((int64_t)lx * ly * lz * 3 > INT32_MAX || ly >= (1 << 22) || lz >= (1 << 22)) {
+      delete B;
+      return bfail(LCEQ_ERR_ARG, "replica %d: lattice too large", r);
+    }
     Replica& R = B->reps[r];
     memset(&R, 0, sizeof(R));
-    R.g.lx = lx;
-    R.g.ly = ly;
-    R.g.lz = lz;
-    R.g.h1 = lx / 2;
-    R.g.h2 = lx * ly / 2;
-    R.g.n = lx * ly * lz / 2;
-    R.g.npad = (R.g.n + 63) & ~63;
+    R.g = geom_init(lx, ly, lz);
     R.nb = (R.g.n + TB_MIN - 1) / TB_MIN;
     R.blk0 = nb_total;
     R.pstride = R.nb;

@@ -75,16 +75,43 @@ struct Geom {
   int lx, ly, lz;
   int h1, h2, n;  // n = sites per parity
   int npad;       // leading dimension of every SoA component array
+  float rh1, rh2;  // 1/h1, 1/h2 (geom_init) for the index divisions
 };
+
+// Geometry of an lx x ly x lz lattice (all even)
+__host__ __device__ inline Geom geom_init(int lx, int ly, int lz) {
+  Geom g;
+  g.lx = lx;
+  g.ly = ly;
+  g.lz = lz;
+  g.h1 = lx / 2;
+  g.h2 = lx * ly / 2;
+  g.n = lx * ly * lz / 2;
+  g.npad = (g.n + 63) & ~63;
+  g.rh1 = 1.0f / (float)g.h1;
+  g.rh2 = 1.0f / (float)g.h2;
+  return g;
+}
+
+// floor(s / d) for s < 2^31 and a quotient below 2^22: the single-precision
+// estimate s * (1/d) is within one of the quotient, and one correction step
+// either way makes it exact (no integer division sequence)
+__device__ __forceinline__ int div_small(int s, int d, float rd) {
+  int q = (int)((float)s * rd);
+  int r = s - q * d;
+  q += r >= d ? 1 : 0;
+  q -= r < 0 ? 1 : 0;
+  return q;
+}
 
 struct Coord {
   int x, y, z;
 };
 
 __device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
-  int z = s / g.h2;
+  int z = div_small(s, g.h2, g.rh2);
   int r = s - z * g.h2;
-  int y = r / g.h1;
+  int y = div_small(r, g.h1, g.rh1);
   int xh = r - y * g.h1;
   return Coord{2 * xh + ((p + y + z) & 1), y, z};
 }
