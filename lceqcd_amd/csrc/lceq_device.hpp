@@ -115,13 +115,28 @@ __device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
   int xh = r - y * g.h1;
   return Coord{2 * xh + ((p + y + z) & 1), y, z};
 }
-__device__ __forceinline__ unsigned site_index(const Geom& g, int x, int y, int z) {
-  return (unsigned)((x >> 1) + g.h1 * y + g.h2 * z);
+// Index changes of the moves +1 (p) and -1 (m) along x, y, z from a site at
+// c, periodic: a y or z move keeps x/2 and changes the index by +-h1 or
+// +-h2 (wrapping at the faces); an x move changes x/2 by 0 or +-1 with the
+// parity of x (wrapping at x = 0, lx - 1).  Moves along different axes add,
+// so the index of c + e_mu - e_nu is s + p[mu] + m[nu].
+struct Steps {
+  int p[3], m[3];
+};
+__device__ __forceinline__ Steps site_steps(const Geom& g, Coord c) {
+  Steps t;
+  bool odd = c.x & 1;
+  t.p[0] = odd ? (c.x == g.lx - 1 ? 1 - g.h1 : 1) : 0;
+  t.m[0] = odd ? 0 : (c.x == 0 ? g.h1 - 1 : -1);
+  t.p[1] = c.y == g.ly - 1 ? -(g.ly - 1) * g.h1 : g.h1;
+  t.m[1] = c.y == 0 ? (g.ly - 1) * g.h1 : -g.h1;
+  t.p[2] = c.z == g.lz - 1 ? -(g.lz - 1) * g.h2 : g.h2;
+  t.m[2] = c.z == 0 ? (g.lz - 1) * g.h2 : -g.h2;
+  return t;
 }
-__device__ __forceinline__ int wrap(int c, int l) { return c < 0 ? c + l : (c >= l ? c - l : c); }
-// index of (c + d), d a displacement with components in {-1,0,1}
-__device__ __forceinline__ unsigned shifted(const Geom& g, Coord c, int dx, int dy, int dz) {
-  return site_index(g, wrap(c.x + dx, g.lx), wrap(c.y + dy, g.ly), wrap(c.z + dz, g.lz));
+// component d (0..2, a run-time value) of a step vector
+__device__ __forceinline__ int step_of(const int (&v)[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
 }
 __device__ __forceinline__ void unit(int mu, int& dx, int& dy, int& dz) {
   dx = mu == 0;

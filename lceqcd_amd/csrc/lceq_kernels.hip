@@ -432,12 +432,10 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
   int sy[3];
   vec3 af;
   if (act) {
-    Coord c = site_coord(g, eo, s);
+    Steps st = site_steps(g, site_coord(g, eo, s));
 #pragma unroll
     for (int mu = 0; mu < 3; ++mu) {
-      int mx, my, mz;
-      unit(mu, mx, my, mz);
-      sy[mu] = shifted(g, c, -mx, -my, -mz);
+      sy[mu] = s + st.m[mu];
       ux[mu] = ld_link(p.Uf, g, mu, eo, s);
       uy[mu] = ld_link(p.Uf, g, mu, oe, sy[mu]);
     }

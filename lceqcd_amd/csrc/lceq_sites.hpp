@@ -38,19 +38,17 @@ struct StapleLinks {
 };
 
 __device__ __forceinline__ void load_staple_links(const double* __restrict__ U, const Geom& g,
-                                                  int eo, int mu, unsigned s, Coord c,
+                                                  int eo, int mu, unsigned s, const Steps& st,
                                                   unsigned sxm, StapleLinks& k) {
   int oe = eo ^ 1;
-  int mx, my, mz;
-  unit(mu, mx, my, mz);
+  int pmu = step_of(st.p, mu);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     int nu = (t == 0) ? (mu == 0 ? 1 : 0) : (mu == 2 ? 1 : 2);
-    int nx, ny, nz;
-    unit(nu, nx, ny, nz);
-    unsigned sxn = shifted(g, c, nx, ny, nz);
-    unsigned sxmn = shifted(g, c, -nx, -ny, -nz);
-    unsigned sxmmn = shifted(g, c, mx - nx, my - ny, mz - nz);
+    int pnu = step_of(st.p, nu), mnu = step_of(st.m, nu);
+    unsigned sxn = (unsigned)((int)s + pnu);
+    unsigned sxmn = (unsigned)((int)s + mnu);
+    unsigned sxmmn = (unsigned)((int)s + pmu + mnu);
     k.mu_xpnu[t] = ld_link(U, g, mu, oe, sxn);
     k.nu_x[t] = ld_link(U, g, nu, eo, s);
     k.nu_xpmu[t] = ld_link(U, g, nu, oe, sxm);
@@ -77,10 +75,9 @@ __device__ __forceinline__ su2 staple_sum(const StapleLinks& k) {
 
 __device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& g, int eo, int mu,
                                       unsigned s, Coord c) {
-  int mx, my, mz;
-  unit(mu, mx, my, mz);
+  Steps st = site_steps(g, c);
   StapleLinks k;
-  load_staple_links(U, g, eo, mu, s, c, shifted(g, c, mx, my, mz), k);
+  load_staple_links(U, g, eo, mu, s, st, (unsigned)((int)s + step_of(st.p, mu)), k);
   return staple_sum(k);
 }
 
@@ -115,10 +112,9 @@ __device__ __forceinline__ void link_load(const LinkSite& p, unsigned s, LinkOpe
   const Geom& g = p.g;
   int eo = p.eo, oe = eo ^ 1, mu = p.mu;
   Coord c = site_coord(g, eo, (int)s);
-  int mx, my, mz;
-  unit(mu, mx, my, mz);
-  unsigned sxm = shifted(g, c, mx, my, mz);
-  load_staple_links(p.U, g, eo, mu, s, c, sxm, o.k);
+  Steps st = site_steps(g, c);
+  unsigned sxm = (unsigned)((int)s + step_of(st.p, mu));
+  load_staple_links(p.U, g, eo, mu, s, st, sxm, o.k);
   o.u = ld_link(p.U, g, mu, eo, s);
   o.ax = ld_higgs(p.A, g, eo, s);
   o.axp = ld_higgs(p.A, g, oe, sxm);
@@ -237,11 +233,10 @@ __device__ __forceinline__ void load_force_links(const double* __restrict__ U,
                                                  const double* __restrict__ A, const Geom& g,
                                                  int eo, unsigned s, Coord c, ForceLinks& k) {
   int oe = eo ^ 1;
+  Steps st = site_steps(g, c);
 #pragma unroll
   for (int mu = 0; mu < 3; ++mu) {
-    int mx, my, mz;
-    unit(mu, mx, my, mz);
-    unsigned sxp = shifted(g, c, mx, my, mz), sxd = shifted(g, c, -mx, -my, -mz);
+    unsigned sxp = (unsigned)((int)s + st.p[mu]), sxd = (unsigned)((int)s + st.m[mu]);
     k.ux[mu] = ld_link(U, g, mu, eo, s);
     k.uy[mu] = ld_link(U, g, mu, oe, sxd);
     k.axp[mu] = ld_higgs(A, g, oe, sxp);
