@@ -76,6 +76,7 @@ struct Geom {
   int h1, h2, n;  // n = sites per parity
   int npad;       // leading dimension of every SoA component array
   float rh1, rh2;  // 1/h1, 1/h2 (geom_init) for the index divisions
+  int wy, wz;      // (ly-1)*h1, (lz-1)*h2: the index steps that wrap y and z
 };
 
 // Geometry of an lx x ly x lz lattice (all even)
@@ -90,6 +91,8 @@ __host__ __device__ inline Geom geom_init(int lx, int ly, int lz) {
   g.npad = (g.n + 63) & ~63;
   g.rh1 = 1.0f / (float)g.h1;
   g.rh2 = 1.0f / (float)g.h2;
+  g.wy = (ly - 1) * g.h1;
+  g.wz = (lz - 1) * g.h2;
   return g;
 }
 
@@ -128,10 +131,10 @@ __device__ __forceinline__ Steps site_steps(const Geom& g, Coord c) {
   bool odd = c.x & 1;
   t.p[0] = odd ? (c.x == g.lx - 1 ? 1 - g.h1 : 1) : 0;
   t.m[0] = odd ? 0 : (c.x == 0 ? g.h1 - 1 : -1);
-  t.p[1] = c.y == g.ly - 1 ? -(g.ly - 1) * g.h1 : g.h1;
-  t.m[1] = c.y == 0 ? (g.ly - 1) * g.h1 : -g.h1;
-  t.p[2] = c.z == g.lz - 1 ? -(g.lz - 1) * g.h2 : g.h2;
-  t.m[2] = c.z == 0 ? (g.lz - 1) * g.h2 : -g.h2;
+  t.p[1] = c.y == g.ly - 1 ? -g.wy : g.h1;
+  t.m[1] = c.y == 0 ? g.wy : -g.h1;
+  t.p[2] = c.z == g.lz - 1 ? -g.wz : g.h2;
+  t.m[2] = c.z == 0 ? g.wz : -g.h2;
   return t;
 }
 // component d (0..2, a run-time value) of a step vector

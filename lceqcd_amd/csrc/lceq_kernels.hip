@@ -42,29 +42,45 @@ struct LinkArgs {
 // Block index in the launch's block order: XCD-aware (contiguous sites per
 // XCD) or the launch order
 __device__ __forceinline__ int shape_block(int nb, int xcd) {
-  return xcd ? xcd_block(nb) : (int)blockIdx.x;
+  int b = xcd_block(nb);  // computed either way: no branch before the loads
+  return xcd ? b : (int)blockIdx.x;
 }
 
 // One site per thread, TBX sites per block
-template <bool KP, bool FED, int TBX>
+template <bool KP, bool FED, int TBX, int MU>
 __global__ __launch_bounds__(TBX) void k_link(LinkArgs a) {
   int blk = shape_block(a.nb, a.xcd);
   unsigned s = (unsigned)(blk * TBX + threadIdx.x);
   double red[3] = {0.0, 0.0, 0.0};
-  if (s < (unsigned)a.site.g.n) link_update_site<KP, FED>(a.site, s, red);
+  if (s < (unsigned)a.site.g.n) {
+    LinkSite p = a.site;
+    p.mu = MU;
+    link_update_site<KP, FED>(p, s, red);
+  }
   if (a.reduce) block_sum_store<3, TBX / 64>(red, a.part, a.pstride, blk);
 }
 
 int shape_blocks(const lceq_lattice* L) { return (L->g.n + L->tb - 1) / L->tb; }
 
-template <bool KP, bool FED>
-static void link_dispatch(lceq_lattice* L, const LinkArgs& a) {
+template <bool KP, bool FED, int MU>
+static void link_dispatch_mu(lceq_lattice* L, const LinkArgs& a) {
   dim3 grid(a.nb);
   switch (L->tb) {
-    case 64: hipLaunchKernelGGL((k_link<KP, FED, 64>), grid, dim3(64), 0, L->stream, a); break;
-    case 128: hipLaunchKernelGGL((k_link<KP, FED, 128>), grid, dim3(128), 0, L->stream, a); break;
-    case 512: hipLaunchKernelGGL((k_link<KP, FED, 512>), grid, dim3(512), 0, L->stream, a); break;
-    default: hipLaunchKernelGGL((k_link<KP, FED, 256>), grid, dim3(256), 0, L->stream, a); break;
+    case 64: hipLaunchKernelGGL((k_link<KP, FED, 64, MU>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_link<KP, FED, 128, MU>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_link<KP, FED, 512, MU>), grid, dim3(512), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((k_link<KP, FED, 256, MU>), grid, dim3(256), 0, L->stream, a); break;
+  }
+}
+
+// The direction is a template argument: the neighbour steps and link offsets
+// of the staple are then fixed at compile time.
+template <bool KP, bool FED>
+static void link_dispatch(lceq_lattice* L, const LinkArgs& a) {
+  switch (a.site.mu) {
+    case 0: link_dispatch_mu<KP, FED, 0>(L, a); break;
+    case 1: link_dispatch_mu<KP, FED, 1>(L, a); break;
+    default: link_dispatch_mu<KP, FED, 2>(L, a); break;
   }
 }
 
@@ -119,81 +135,98 @@ struct FinalizeArgs {
 // One 1024-thread block sums every partial the closing routines wrote: six
 // link slots x 3 quantities and two Higgs slots x 4 quantities.  Wave w owns
 // quantities w and w+16; each lane sums its strided share of the block
-// partials (all loads in flight at once), a xor-butterfly finishes the wave
-// sum (fixed order, so the result is reproducible), and thread 0 combines the
-// slots in the reference's loop order: f/agauge.f:195-200 (links),
-// f/ahiggs1.f:107-123 and f/ahiggs2.f:138-143 (Higgs).
+// partials (the loads of both quantities in flight at once), a xor-butterfly
+// finishes the wave sum (fixed order, so the result is reproducible), and
+// thread 0 combines the slots in the reference's loop order: f/agauge.f:195-200
+// (links), f/ahiggs1.f:107-123 and f/ahiggs2.f:138-143 (Higgs).  The
+// observables are read by wave 0 alongside the partials, updated in LDS and
+// written back (and to the history row) by 16 lanes at once.
 constexpr int FIN_TB = 1024;
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs p) {
   constexpr int NV = 6 * 3 + 2 * 4;
   __shared__ double tot[32];
+  __shared__ double ob[LCEQ_OBS_N];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double ov = threadIdx.x < LCEQ_OBS_N ? p.obs[threadIdx.x] : 0.0;
+  const double* x[2];
+  int nbq[2];
+  bool on[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     int q = w + 16 * h;  // 0..17 link (slot*3 + quantity), 18..25 Higgs
-    if (q >= NV) continue;
-    bool on = q < 18 ? p.link_kind != 0 : p.higgs_kind != 0;
-    double t = 0.0;
-    if (on) {
-      size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
-                          : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
-      const double* x = p.part + row * p.pstride;
-      int nb = q < 18 ? p.nb_link : p.nb_higgs;
-      for (int i0 = lane; i0 < nb; i0 += 64 * 8) {
-        double y[8];  // eight loads in flight before the adds
+    on[h] = q < NV && (q < 18 ? p.link_kind != 0 : p.higgs_kind != 0);
+    size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
+                        : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
+    x[h] = p.part + row * p.pstride;
+    nbq[h] = on[h] ? (q < 18 ? p.nb_link : p.nb_higgs) : 0;
+  }
+  double t[2] = {0.0, 0.0};
+  for (int i0 = lane; i0 < max(nbq[0], nbq[1]); i0 += 64 * 8) {
+    double y[2][8];  // sixteen loads in flight before the adds
 #pragma unroll
-        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < nb ? x[i0 + 64 * u] : 0.0;
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) t += y[u];
+      for (int u = 0; u < 8; ++u) y[h][u] = i0 + 64 * u < nbq[h] ? x[h][i0 + 64 * u] : 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[h] += y[h][u];
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t[h] += __shfl_xor(t[h], off, 64);
+    if (lane == 0 && w + 16 * h < NV) tot[w + 16 * h] = t[h];
+  }
+  if (threadIdx.x < LCEQ_OBS_N) ob[threadIdx.x] = ov;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double* v = tot;
+    double* obs = ob;
+    if (p.link_kind) {
+      double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
+      for (int slot = 0; slot < 6; ++slot) {
+        plaq = plaq + v[slot * 3];
+        nacch += v[slot * 3 + 1];
+        acc1 += v[slot * 3 + 2];
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      obs[OBS_NACCH] = nacch;
+      obs[OBS_NACC1] = acc1;
+      if (p.link_meas == 1) obs[OBS_PLAQ0] = plaq / (12.0 * p.vol);
+      double accept;
+      if (p.link_kind == 1) {  // (nacch/3V)(acc1/12V) with measflag, else nacch
+        accept = nacch;
+        if (p.link_meas == 1) accept = (accept / (3.0 * p.vol)) * (acc1 / (12.0 * p.vol));
+      } else {
+        accept = acc1 / (3.0 * p.vol);
+      }
+      obs[OBS_ACC_LINK] = accept;
     }
-    if (lane == 0) tot[q] = t;
+    if (p.higgs_kind) {
+      double acc = v[18] + v[22];
+      double r2 = v[19] + v[23], r4 = v[20] + v[24], kin = v[21] + v[25];
+      if (p.higgs_meas == 1) {
+        obs[OBS_A02] = r2 / p.vol;
+        obs[OBS_A04] = r4 / p.vol;
+        obs[OBS_A0KIN] = kin / (6.0 * p.vol);
+      }
+      double accept = acc / p.vol;
+      if (p.higgs_kind == 1) {
+        obs[OBS_ACC_HIGGS] = accept;
+        double d = obs[OBS_DELTA];
+        if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+        obs[OBS_DELTA] = d;
+      } else if (p.higgs_meas == 1) {
+        obs[OBS_ACC_HIGGS] = accept;
+      }
+    }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double* v = tot;
-  double* obs = p.obs;
-  if (p.link_kind) {
-    double plaq = 0.0, nacch = 0.0, acc1 = 0.0;
-    for (int slot = 0; slot < 6; ++slot) {
-      plaq = plaq + v[slot * 3];
-      nacch += v[slot * 3 + 1];
-      acc1 += v[slot * 3 + 2];
-    }
-    obs[OBS_NACCH] = nacch;
-    obs[OBS_NACC1] = acc1;
-    if (p.link_meas == 1) obs[OBS_PLAQ0] = plaq / (12.0 * p.vol);
-    double accept;
-    if (p.link_kind == 1) {  // (nacch/3V)(acc1/12V) with measflag, else nacch
-      accept = nacch;
-      if (p.link_meas == 1) accept = (accept / (3.0 * p.vol)) * (acc1 / (12.0 * p.vol));
-    } else {
-      accept = acc1 / (3.0 * p.vol);
-    }
-    obs[OBS_ACC_LINK] = accept;
+  if (threadIdx.x < LCEQ_OBS_N) {
+    double o = ob[threadIdx.x];
+    p.obs[threadIdx.x] = o;
+    if (p.hist_row) p.hist_row[threadIdx.x] = o;
   }
-  if (p.higgs_kind) {
-    double acc = v[18] + v[22];
-    double r2 = v[19] + v[23], r4 = v[20] + v[24], kin = v[21] + v[25];
-    if (p.higgs_meas == 1) {
-      obs[OBS_A02] = r2 / p.vol;
-      obs[OBS_A04] = r4 / p.vol;
-      obs[OBS_A0KIN] = kin / (6.0 * p.vol);
-    }
-    double accept = acc / p.vol;
-    if (p.higgs_kind == 1) {
-      obs[OBS_ACC_HIGGS] = accept;
-      double d = obs[OBS_DELTA];
-      if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
-      obs[OBS_DELTA] = d;
-    } else if (p.higgs_meas == 1) {
-      obs[OBS_ACC_HIGGS] = accept;
-    }
-  }
-  if (p.hist_row)
-    for (int i = 0; i < LCEQ_OBS_N; ++i) p.hist_row[i] = obs[i];
 }
 
 void launch_finalize(lceq_lattice* L, int link_kind, int link_meas, int higgs_kind,

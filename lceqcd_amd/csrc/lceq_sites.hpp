@@ -293,6 +293,9 @@ __device__ __forceinline__ void higgs_update_site(const HiggsSite& p, unsigned s
   load_force_links(p.U, p.A, g, eo, s, c, k);
   vec3 a = ld_higgs(p.A, g, eo, s), an;
   double fu[7];
+  // ahiggs1's step size lives on the device (it self-tunes there); its load
+  // goes out with the site's operand loads, not after them
+  double delta = KIND == 1 ? *p.delta : 0.0;
   if (FED) {
     if (KIND == 1) {
       fu[0] = p.fed[3 * s];
@@ -310,7 +313,7 @@ __device__ __forceinline__ void higgs_update_site(const HiggsSite& p, unsigned s
   vec3 F = force_sum(k);
   double ro = 0.0, rn = 0.0, boltz, racc;
   if (KIND == 1) {  // f/ahiggs1.f:51-92
-    double delta = *p.delta, r[3];
+    double r[3];
     if (FED) {
       r[0] = fu[0];
       r[1] = fu[1];
