@@ -74,7 +74,7 @@ SWEEP_BYTES_PER_SITE = 6 * LINK_PASS_BYTES_PER_SITE * 5 + 2 * HIGGS_PASS_BYTES_P
 # (XCD-aware order), FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
 # 512 MiB device copy), measured with rocprofv3 --pmc
 # (profiles/k_link_or_64cubed_hbm_traffic.md)
-OR_TRAFFIC_64 = {512: 33.69e6 + 4.03e6, 256: 33.73e6 + 4.04e6}
+OR_TRAFFIC_64 = {512: 33.67e6 + 4.03e6, 256: 33.72e6 + 4.04e6}
 
 
 def couplings(beta_a_trace=0.33):
@@ -395,6 +395,25 @@ def learned(args, L0):
     print(json.dumps(rec), flush=True)
 
 
+def copy_rate(nbytes, reps):
+    """Read+write GB/s of the library's plain device copy of nbytes."""
+    import ctypes
+    from lceqcd_amd import _lib
+    us = ctypes.c_double()
+    nbytes = max(16, int(nbytes) // 16 * 16)
+    _lib.check(_lib.load().lceq_copy_bandwidth(nbytes, reps, ctypes.byref(us)))
+    return 2 * nbytes / (us.value * 1e-6) / 1e9
+
+
+def copy_ceiling(launch_bytes, achieved):
+    """The practical bandwidth beside the 8 TB/s peak: a plain copy that
+    moves the launch's own bytes (half read, half written), and one of 1 GiB,
+    which no cache holds."""
+    same = copy_rate(launch_bytes / 2, 200)
+    return {"same_bytes_GBps": same, "hbm_resident_1GiB_GBps": copy_rate(1 << 30, 20),
+            "achieved_frac_of_same_bytes_copy": achieved / same}
+
+
 def calibrate_copy():
     import torch
     n = 64 * 1024 * 1024  # 64 Mi doubles = 512 MiB per buffer
@@ -522,6 +541,7 @@ def main():
             "avg_launch_us": avg_s * 1e6,
             "launches": or_n,
             "kp_avg_launch_us": kp_ms * 1e3 / max(kp_n, 1),
+            "copy_ceiling": copy_ceiling(launch_bytes, achieved),
         },
         "observables_last_sweep": {"plaq0": hist[-1, 0], "a02": hist[-1, 2],
                                    "a04": hist[-1, 3], "a0kin": hist[-1, 1],

@@ -88,6 +88,13 @@ int lceq_get_shape(lceq_lattice *L, int *threads_per_block, int *xcd_order);
  * counter and the history are restored.  ms (or NULL) receives 8 times per
  * sweep, shapes (64,128,256,512) x (launch order, XCD order).  Philox only. */
 int lceq_autotune(lceq_lattice *L, int n_or, int trials, double *ms);
+/* Measurement aid with no counterpart in the reference: a plain device copy
+ * of a `bytes` buffer (a multiple of 16; one 16-byte element per thread),
+ * 3 untimed copies, then `reps` copies timed with HIP events.  us_per_copy
+ * receives the average; 2*bytes/us_per_copy is the practical read+write
+ * bandwidth that the lattice launches are compared with.  Allocates and frees
+ * two buffers of `bytes`. */
+int lceq_copy_bandwidth(int64_t bytes, int reps, double *us_per_copy);
 
 /* ---- uniforms --------------------------------------------------------- */
 /* Counter-based Philox4x32-10 streams: draw k of site s in update pass P uses

@@ -19,6 +19,7 @@ SIGNATURES = {
     "lceq_set_shape": [VP, I, I],
     "lceq_get_shape": [VP, IP, IP],
     "lceq_autotune": [VP, I, I, DP],
+    "lceq_copy_bandwidth": [I64, I, DP],
     "lceq_rng_philox": [VP, U64, U64],
     "lceq_rng_feed": [VP, DP, I64],
     "lceq_rng_state": [VP, C.POINTER(I64), C.POINTER(U64)],

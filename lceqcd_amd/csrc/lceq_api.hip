@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <string>
 #include <vector>
 
@@ -189,6 +190,44 @@ extern "C" int lceq_get_shape(lceq_lattice* L, int* threads_per_block, int* xcd_
   if (threads_per_block) *threads_per_block = L->tb;
   if (xcd_order) *xcd_order = L->xcd;
   return LCEQ_OK;
+}
+
+// Plain device copy, one 16-byte element per thread
+__global__ __launch_bounds__(256) void k_copy16(const double2* __restrict__ a,
+                                                double2* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+extern "C" int lceq_copy_bandwidth(int64_t bytes, int reps, double* us_per_copy) {
+  if (!us_per_copy) return fail(LCEQ_ERR_ARG, "null output pointer");
+  if (bytes < 16 || bytes % 16 || reps < 1 || (bytes / 16 + 255) / 256 > INT32_MAX)
+    return fail(LCEQ_ERR_ARG, "copy size must be a positive multiple of 16 below 2^43, reps >= 1");
+  int64_t n = bytes / 16;
+  int nb = (int)((n + 255) / 256);
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LCEQ_OK;
+  float el = 0.0f;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+      hipMemset(a, 0, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(LCEQ_ERR_HIP, "copy buffers or events could not be set up");
+  } else {
+    for (int r = 0; r < 3; ++r) hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy16, dim3(nb), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess)
+      rc = fail(LCEQ_ERR_HIP, "copy timing failed");
+    else
+      *us_per_copy = 1e3 * (double)el / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 // Time `trials` production sweeps (schedule 0, no measurement) at every

@@ -52,6 +52,14 @@ def test_null_handle_is_rejected(lib):
     assert lib.lceq_landau_gauge(None, None, None) == -1
 
 
+def test_copy_bandwidth_arguments_checked_before_device_use(lib):
+    us = C.c_double(-1.0)
+    for nbytes, reps in ((0, 5), (24, 5), (1 << 20, 0)):
+        assert lib.lceq_copy_bandwidth(nbytes, reps, C.byref(us)) == -1
+    assert lib.lceq_copy_bandwidth(1 << 20, 5, None) == -1
+    assert us.value == -1.0
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     from lceqcd_amd import _lib
     monkeypatch.setattr(_lib, "_lib", None)

@@ -43,3 +43,17 @@ def test_autotune_restores_the_chain():
     assert ms[got] == min(ms.values())
     assert np.array_equal(u0, u1) and np.array_equal(a0, a1)
     np.testing.assert_allclose(h1, h0, rtol=1e-13, atol=1e-15)
+
+
+def test_copy_bandwidth_probe():
+    """The plain-copy probe that the benchmark reports beside the 8 TB/s peak
+    returns a finite time and a rate below that peak for a 64 MiB copy."""
+    import ctypes as C
+    from lceqcd_amd import _lib
+    us = C.c_double()
+    nbytes = 64 << 20
+    print("copy probe", nbytes, flush=True)
+    _lib.check(_lib.load().lceq_copy_bandwidth(nbytes, 10, C.byref(us)))
+    rate = 2 * nbytes / (us.value * 1e-6) / 1e9
+    print("copy probe GB/s", rate, flush=True)
+    assert np.isfinite(us.value) and 0.0 < rate < 20000.0
