@@ -440,7 +440,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64"])
-    ap.add_argument("--sweeps", type=int, default=20000)
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="sweeps per chain (learned workloads; default 60000 at 48^3, "
+                         "20000 at 96^3: a few % statistical error on tau_int)")
     ap.add_argument("--train-configs", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--therm", type=int, default=300,
@@ -461,6 +463,8 @@ def main():
         return replicas96(args)
     if args.workload == "refsched64":
         return refsched64(args)
+    if args.sweeps is None:
+        args.sweeps = 60000 if args.workload == "mlp48" else 20000
     if args.workload in ("mlp48", "mlp96"):
         return learned(args, int(args.workload[3:]))
 
