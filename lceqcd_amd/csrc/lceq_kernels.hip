@@ -505,8 +505,7 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       o.a = c_mk(cos(tho), -t2);
       o.b = c_mk(-(f * o.b.re), -(f * o.b.im));
     }
-    su2 t = m_nn(o, m_ad(af, o));  // :243-252
-    st_higgs(p.Af, g, eo, s, vec3{{t.b.im, t.b.re, t.a.im}});
+    st_higgs(p.Af, g, eo, s, adj_rot<false>(o, af));  // g X(a_f) g^+, :243-252
 #pragma unroll
     for (int mu = 0; mu < 3; ++mu) {  // :256-324
       su2 nx = m_nn(o, ux[mu]);

@@ -81,9 +81,28 @@ __device__ __forceinline__ su2 staple(const double* __restrict__ U, const Geom& 
   return staple_sum(k);
 }
 
-__device__ __forceinline__ double hop_action(double beta_a, const vec3& ax, const vec3& axp,
-                                             su2 u) {
-  return beta_a * m_nn(m_an(ax, u), m_ad(axp, u)).a.re;  // f/agauge.f:126-159
+// The hopping action beta_a Re[(X(a) U)(X(b) U^+)]_11 of f/agauge.f:126-159,
+// which is -beta_a a.R(U) b (R as in adj_rot):
+//   a.R b = (c0^2 - |c|^2)(a.b) + 2 (c.a)(c.b) - 2 c0 c.(b x a),
+// equal to the products in exact arithmetic.  a.b and b x a do not depend on
+// U and serve the old and the new link.
+struct HopShared {
+  double ab;
+  double bxa[3];
+};
+__device__ __forceinline__ HopShared hop_shared(const vec3& a, const vec3& b) {
+  return HopShared{(a.x[0] * b.x[0] + a.x[1] * b.x[1]) + a.x[2] * b.x[2],
+                   {b.x[1] * a.x[2] - b.x[2] * a.x[1], b.x[2] * a.x[0] - b.x[0] * a.x[2],
+                    b.x[0] * a.x[1] - b.x[1] * a.x[0]}};
+}
+__device__ __forceinline__ double hop_action(double beta_a, const vec3& a, const vec3& b,
+                                             const HopShared& h, su2 u) {
+  double c0 = u.a.re, c1 = u.b.im, c2 = u.b.re, c3 = u.a.im;
+  double s = c0 * c0 - ((c1 * c1 + c2 * c2) + c3 * c3);
+  double ca = (c1 * a.x[0] + c2 * a.x[1]) + c3 * a.x[2];
+  double cb = (c1 * b.x[0] + c2 * b.x[1]) + c3 * b.x[2];
+  double tr = (c1 * h.bxa[0] + c2 * h.bxa[1]) + c3 * h.bxa[2];
+  return -beta_a * ((s * h.ab + 2.0 * (ca * cb)) - 2.0 * (c0 * tr));
 }
 
 // U / |U| (f/renorm.f:24-34) with the reciprocal square root (within an ulp
@@ -200,8 +219,9 @@ __device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const
     su2 r = m_nn(m_nn(vh, u), vh);
     h = su2{c_conj(r.a), c_neg(r.b)};
   }
-  double sa_o = hop_action(p.beta_a, o.ax, o.axp, u);
-  double sa_n = hop_action(p.beta_a, o.ax, o.axp, h);
+  HopShared hs = hop_shared(o.ax, o.axp);
+  double sa_o = hop_action(p.beta_a, o.ax, o.axp, hs, u);
+  double sa_n = hop_action(p.beta_a, o.ax, o.axp, hs, h);
   if (exp(sa_o - sa_n) >= o.racc) {  // f/agauge.f:167-174
     u = p.renorm ? normalized(h) : h;
     st_link(p.U, p.g, p.mu, p.eo, s, u);
@@ -244,16 +264,16 @@ __device__ __forceinline__ void load_force_links(const double* __restrict__ U,
   }
 }
 
-// F = sum_mu extract[U(x) X(a(x+mu)) U(x)^+ + U(x-mu)^+ X(a(x-mu)) U(x-mu)]
+// F = sum_mu extract[U(x) X(a(x+mu)) U(x)^+ + U(x-mu)^+ X(a(x-mu)) U(x-mu)],
+// each term as the rotation adj_rot
 __device__ __forceinline__ vec3 force_sum(const ForceLinks& k) {
   vec3 F{{0.0, 0.0, 0.0}};
 #pragma unroll
   for (int mu = 0; mu < 3; ++mu) {
-    su2 t1 = m_nn(k.ux[mu], m_ad(k.axp[mu], k.ux[mu]));
-    su2 t2 = m_dn(k.uy[mu], m_an(k.axd[mu], k.uy[mu]));
-    F.x[2] = F.x[2] + c_add(t1.a, t2.a).im;
-    F.x[1] = F.x[1] + c_add(t1.b, t2.b).re;
-    F.x[0] = F.x[0] + c_add(t1.b, t2.b).im;
+    vec3 t1 = adj_rot<false>(k.ux[mu], k.axp[mu]);
+    vec3 t2 = adj_rot<true>(k.uy[mu], k.axd[mu]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) F.x[i] = F.x[i] + (t1.x[i] + t2.x[i]);
   }
   return F;
 }
