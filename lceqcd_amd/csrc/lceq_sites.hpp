@@ -216,8 +216,13 @@ __device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const
     w.b = c_mk(a0b * o.kp_sin, a0b * o.kp_cos);
     h = m_nd(w, vh);  // f/agauge.f:98
   } else {
-    su2 r = m_nn(m_nn(vh, u), vh);
-    h = su2{c_conj(r.a), c_neg(r.b)};
+    // U -> Vhat^+ U^+ Vhat^+.  For a unit Vhat this is the reflection
+    // 2 <U,W> W - U of U about W = Vhat^+ in the 4-vectors (Re a, Im a, Re b,
+    // Im b), with <U,W> = Re(U Vhat)_11: 9 operations instead of two products
+    double d2 = 2.0 * ((u.a.re * vh.a.re - u.a.im * vh.a.im) -
+                       (u.b.re * vh.b.re + u.b.im * vh.b.im));
+    h = su2{c_mk(d2 * vh.a.re - u.a.re, -(d2 * vh.a.im) - u.a.im),
+            c_mk(-(d2 * vh.b.re) - u.b.re, -(d2 * vh.b.im) - u.b.im)};
   }
   HopShared hs = hop_shared(o.ax, o.axp);
   double sa_o = hop_action(p.beta_a, o.ax, o.axp, hs, u);
