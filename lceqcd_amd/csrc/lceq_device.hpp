@@ -29,22 +29,45 @@ struct su2 {
   cpx a, b;
 };
 
-// products in the operand conventions of f/matmul.f
+// Products in the operand conventions of f/matmul.f, written in the real
+// components (Re a, Im a, Re b, Im b): each output is a sum of four products,
+// one multiply and three fused multiply-adds (16 operations per product; the
+// complex form costs 20).  DP / DQ take the adjoint of p / q, which flips the
+// signs of Im a, Re b and Im b.
+template <bool DP, bool DQ>
+__device__ __forceinline__ su2 q_prod(su2 p, su2 q) {
+  double p0 = p.a.re, p1 = DP ? -p.a.im : p.a.im, p2 = DP ? -p.b.re : p.b.re,
+         p3 = DP ? -p.b.im : p.b.im;
+  double q0 = q.a.re, q1 = DQ ? -q.a.im : q.a.im, q2 = DQ ? -q.b.re : q.b.re,
+         q3 = DQ ? -q.b.im : q.b.im;
+  return su2{c_mk(((p0 * q0 - p1 * q1) - p2 * q2) - p3 * q3,
+                  ((p0 * q1 + p1 * q0) + p2 * q3) - p3 * q2),
+             c_mk(((p0 * q2 - p1 * q3) + p2 * q0) + p3 * q1,
+                  ((p0 * q3 + p1 * q2) - p2 * q1) + p3 * q0)};
+}
+// acc + p q, the chains started at acc (no separate additions)
+template <bool DP, bool DQ>
+__device__ __forceinline__ su2 q_prod_acc(su2 acc, su2 p, su2 q) {
+  double p0 = p.a.re, p1 = DP ? -p.a.im : p.a.im, p2 = DP ? -p.b.re : p.b.re,
+         p3 = DP ? -p.b.im : p.b.im;
+  double q0 = q.a.re, q1 = DQ ? -q.a.im : q.a.im, q2 = DQ ? -q.b.re : q.b.re,
+         q3 = DQ ? -q.b.im : q.b.im;
+  return su2{c_mk((((acc.a.re + p0 * q0) - p1 * q1) - p2 * q2) - p3 * q3,
+                  (((acc.a.im + p0 * q1) + p1 * q0) + p2 * q3) - p3 * q2),
+             c_mk((((acc.b.re + p0 * q2) - p1 * q3) + p2 * q0) + p3 * q1,
+                  (((acc.b.im + p0 * q3) + p1 * q2) - p2 * q1) + p3 * q0)};
+}
 __device__ __forceinline__ su2 m_nn(su2 p, su2 q) {  // U1 U2   (matmul.f:22-23)
-  return su2{c_sub(c_mul(p.a, q.a), c_mul(p.b, c_conj(q.b))),
-             c_add(c_mul(p.a, q.b), c_mul(p.b, c_conj(q.a)))};
+  return q_prod<false, false>(p, q);
 }
 __device__ __forceinline__ su2 m_dn(su2 p, su2 q) {  // U1^+ U2 (matmul.f:52-56)
-  return su2{c_add(c_mul(c_conj(p.a), q.a), c_mul(p.b, c_conj(q.b))),
-             c_sub(c_mul(c_conj(p.a), q.b), c_mul(p.b, c_conj(q.a)))};
+  return q_prod<true, false>(p, q);
 }
 __device__ __forceinline__ su2 m_nd(su2 p, su2 q) {  // U1 U2^+ (matmul.f:85-89)
-  return su2{c_add(c_mul(p.a, c_conj(q.a)), c_mul(p.b, c_conj(q.b))),
-             c_add(c_neg(c_mul(p.a, q.b)), c_mul(p.b, q.a))};
+  return q_prod<false, true>(p, q);
 }
 __device__ __forceinline__ su2 m_dd(su2 p, su2 q) {  // U1^+ U2^+ (matmul.f:118-122)
-  return su2{c_sub(c_mul(c_conj(p.a), c_conj(q.a)), c_mul(p.b, c_conj(q.b))),
-             c_sub(c_neg(c_mul(c_conj(p.a), q.b)), c_mul(p.b, q.a))};
+  return q_prod<true, true>(p, q);
 }
 struct vec3 {
   double x[3];

@@ -58,17 +58,16 @@ __device__ __forceinline__ void load_staple_links(const double* __restrict__ U, 
   }
 }
 
-// V = sum_t [ 4 3^+ 2^+ + 7^+ 6^+ 5 ] in the accumulation order of f/staples.f:210-216
+// V = sum_t [ 4 3^+ 2^+ + 7^+ 6^+ 5 ] in the accumulation order of f/staples.f:210-216,
+// each outer product accumulated into V by its own multiply-add chain
 __device__ __forceinline__ su2 staple_sum(const StapleLinks& k) {
   su2 V{c_mk(0.0, 0.0), c_mk(0.0, 0.0)};
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     su2 up1 = m_dd(k.mu_xpnu[t], k.nu_x[t]);
-    su2 up2 = m_nn(k.nu_xpmu[t], up1);
     su2 bt1 = m_dn(k.mu_xmnu[t], k.nu_xmnu[t]);
-    su2 bt2 = m_dn(k.nu_xpmumnu[t], bt1);
-    V.a = c_add(c_add(V.a, up2.a), bt2.a);
-    V.b = c_add(c_add(V.b, up2.b), bt2.b);
+    V = q_prod_acc<false, false>(V, k.nu_xpmu[t], up1);
+    V = q_prod_acc<true, false>(V, k.nu_xpmumnu[t], bt1);
   }
   return V;
 }
