@@ -74,7 +74,8 @@ SWEEP_BYTES_PER_SITE = 6 * LINK_PASS_BYTES_PER_SITE * 5 + 2 * HIGGS_PASS_BYTES_P
 # (XCD-aware order), FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from a
 # 512 MiB device copy), measured with rocprofv3 --pmc
 # (profiles/k_link_or_64cubed_hbm_traffic.md)
-OR_TRAFFIC_64 = {512: 33.67e6 + 4.03e6, 256: 33.72e6 + 4.04e6}
+OR_TRAFFIC_64 = {512: 33.66e6 + 4.03e6, 256: 33.68e6 + 4.04e6, 128: 33.68e6 + 4.04e6,
+                 64: 33.65e6 + 4.04e6}
 
 
 def couplings(beta_a_trace=0.33):
