@@ -455,7 +455,6 @@ __device__ __forceinline__ bool gfix_stop(double da, double iter, double limit_o
 // sum in the same order, so all blocks agree) and the launch returns when
 // the loop has ended; the operand loads are issued before the test.
 __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
-  if (p.guarded && p.obs[OBS_GF_DONE] != 0.0) return;  // uniform over the grid
   const Geom& g = p.g;
   int blk = xcd_block(p.nb);
   int s = blk * TB + threadIdx.x;
@@ -474,6 +473,9 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
     }
     af = ld_higgs(p.Af, g, eo, s);
   }
+  // uniform over the grid; tested after the operand loads are issued, so that
+  // its round trip overlaps theirs
+  if (p.guarded && p.obs[OBS_GF_DONE] != 0.0) return;
   if (p.check) {
     double da = gfix_da(p.part, p.nb, (double)g.n);
     if (gfix_stop(da, p.iter, p.limit_or, p.max_iter, p.obs, blk == 0 && threadIdx.x == 0))
@@ -494,8 +496,8 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
       red[0] = d * d + (w.b.re * w.b.re + w.b.im * w.b.im);
     }
     double k = (w.a.re * w.a.re + w.a.im * w.a.im) + (w.b.re * w.b.re + w.b.im * w.b.im);
-    double sk = sqrt(k);  // :169-183
-    su2 o{c_mk(w.a.re / sk, w.a.im / sk), c_mk(w.b.re / sk, w.b.im / sk)};
+    double rk = rsqrt(k);  // w / sqrt(k), :169-183
+    su2 o{c_scale(rk, w.a), c_scale(rk, w.b)};
     double ra = fmin(1.0, fmax(-1.0, o.a.re));
     double th = acos(ra);
     if (th != 0.0) {
