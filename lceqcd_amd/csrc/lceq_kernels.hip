@@ -502,7 +502,9 @@ __global__ __launch_bounds__(256) void k_gfix(GfArgs p) {
     double th = acos(ra);
     if (th != 0.0) {
       double tho = p.lambda * th;
-      double f = sin(tho) / sin(th);
+      // sin(th) for th = acos(ra) in (0, pi] is sqrt((1 - ra)(1 + ra)); the
+      // factored form keeps full relative precision as th -> 0
+      double f = sin(tho) * rsqrt((1.0 - ra) * (1.0 + ra));
       double t2 = f * o.a.im;
       o.a = c_mk(cos(tho), -t2);
       o.b = c_mk(-(f * o.b.re), -(f * o.b.im));
