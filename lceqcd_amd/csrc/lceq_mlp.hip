@@ -306,9 +306,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     philox_pair(p.key, sl, 0, p.pass, 1, u2, u3);
     philox_pair(p.key, sl, 0, p.pass, 2, racc, dummy);
     double r01 = sqrt(-2.0 * log(u0)), r23 = sqrt(-2.0 * log(u2));
-    z[0] = r01 * cos(TPI * u1);
-    z[1] = r01 * sin(TPI * u1);
-    z[2] = r23 * cos(TPI * u3);
+    double s1, c1;
+    sincospi(2.0 * u1, &s1, &c1);  // sin, cos(2 pi u1) without a rounded 2 pi
+    z[0] = r01 * c1;
+    z[1] = r01 * s1;
+    z[2] = r23 * cospi(2.0 * u3);
   }
   vec3 F = force_sum(k);
   Frame fr = make_frame(F.x);

@@ -391,7 +391,7 @@ __device__ __forceinline__ void higgs_update_site(const HiggsSite& p, unsigned s
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         double w1 = -1.0 * (1.0 / width) * log(w[2 * i]);
-        double gi = sqrt(w1) * cos(TPI * w[2 * i + 1]);
+        double gi = sqrt(w1) * cospi(2.0 * w[2 * i + 1]);  // cos(2 pi w), as in the KP step
         an.x[i] = gi + p.beta_a * F.x[i] * cx;
       }
     } else {  // f/ahiggs3.f:83-87
