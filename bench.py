@@ -14,7 +14,10 @@ step), with plaquette / <rho> / <rho^2> / <a.F> accumulated every sweep.
 Before the timed region the autotuner picks the launch shape (threads per
 block, XCD-aware block order) from 10 sweeps per shape and restores the chain
 (--shape T,X fixes it instead, e.g. for profiling).
-Prints one JSON line.  value = site-updates/s (V sites per sweep).
+Prints one JSON line.  value = site-updates/s (V sites per sweep).  The
+roofline object also carries copy_ceiling: the rate of the library's plain
+device copy (lceq_copy_bandwidth) moving the OR launch's own bytes, and of a
+1 GiB copy, timed in the same run.
 
     python bench.py --workload replicas96 [--steps K] [--hist-out FILE]
 
@@ -26,7 +29,7 @@ the same launches (replayed hipGraphs of 8 sweeps, launch shape picked by
 the autotuner, which leaves the chains unchanged); value = aggregate
 site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
 
-    python bench.py --workload mlp48 | mlp96 [--sweeps K]
+    python bench.py --workload mlp48 | mlp96 [--sweeps K]   (K: 60000 at 48^3, 20000 at 96^3)
 
 the learned Higgs proposal at 48^3 or 96^3 near the transition (trace
 convention beta_G = 9, beta_A = 0.34, beta_2 = -0.9, beta_4 = 0.1): a network
