@@ -209,6 +209,46 @@ def cpu_baseline_reference(L, V, seconds):
                       "%.1f s timed inside oracle/_ref/refharness64_64" % ((k,) + L.shape + (tk,))}
 
 
+def cpu_baseline_reference_slabs(L, seconds, procs):
+    """The reference's own routines on `procs` host cores at once: process p
+    sweeps z-slab p of the lattice's current configuration (lz/procs planes,
+    the share of one rank in a z-decomposition over `procs` ranks) with
+    periodic wrap in z instead of a halo exchange, so this is an upper bound
+    of the reference's `procs`-rank MPI throughput.  About `seconds` of
+    sweeps; None when the reference build is absent or lz % procs != 0."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    from oracle import refrun
+    lx, ly, lz = L.shape
+    if not os.access(refrun.big_harness_path(64), os.X_OK) or lz % procs or (lz // procs) % 2:
+        return None
+    u, a = L.get_links(), L.get_higgs()
+    dz = lz // procs
+    h2 = lx * ly // 2
+    shape = (lx, ly, dz)
+    slabs = [(np.ascontiguousarray(u[:, :, h2 * dz * p:h2 * dz * (p + 1), :]),
+              np.ascontiguousarray(a[:, h2 * dz * p:h2 * dz * (p + 1), :])) for p in range(procs)]
+    from lceqcd_amd import couplings_from_xy
+    ba, b2, b4 = couplings_from_xy(9.0, 0.1, 0.5)
+    kw = dict(prec=64, beta=9.0, beta_a=ba, beta_2=b2, beta_4=b4, imode=0)
+
+    def run_all(k):
+        with ThreadPoolExecutor(max_workers=procs) as ex:
+            return list(ex.map(lambda sl: refrun.time_sweeps(shape, sl[0], sl[1], k, **kw)[0],
+                               slabs))
+    t1 = max(run_all(1))
+    k = max(1, min(2000, int(seconds / max(t1, 1e-3))))
+    tk = max(run_all(k))
+    V = lx * ly * lz
+    return {"value": V * k / tk, "unit": "site-updates/s", "cores": procs, "kind": "reference",
+            "sample": "%d concurrent processes of the reference's own f/main.f:37-41 routines "
+                      "(ahiggs2, ahiggs3, agauge, renorm; 8-byte reals), each sweeping one "
+                      "%dx%dx%d z-slab of the same %dx%dx%d configuration %d times with periodic "
+                      "wrap instead of the halo exchange: an upper bound of the reference's "
+                      "%d-rank MPI throughput; slowest process %.1f s inside "
+                      "oracle/_ref/refharness64_64" % ((procs,) + shape + L.shape + (k, procs, tk))}
+
+
 def refsched64(args):
     """The reference schedule, gauge fixing and correlators at 64^3."""
     import lceqcd_amd
@@ -308,7 +348,12 @@ def refsched64(args):
     }
     if not args.no_cpu:
         cb = cpu_baseline_reference(L, V, args.cpu_seconds)
-        if cb is not None:
+        cm = cpu_baseline_reference_slabs(L, args.cpu_seconds, args.cpu_procs)
+        if cm is not None:
+            rec["cpu_baseline"] = cm
+            if cb is not None:
+                rec["config"]["reference_one_rank"] = cb
+        elif cb is not None:
             rec["cpu_baseline"] = cb
     print(json.dumps(rec), flush=True)
 
@@ -441,6 +486,8 @@ def main():
     ap.add_argument("--lattice", default="64,64,64")
     ap.add_argument("--n-or", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="host cores for the reference's multi-process baseline (refsched64)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64"])
