@@ -111,6 +111,47 @@ def cpu_baseline(shape, seconds, n_or):
                       "oracle/lceq_oracle.c single-threaded" % (sweeps, n_or, *shape, el)}
 
 
+def cpu_baseline_slabs(shape, seconds, n_or, procs):
+    """The CPU restatement on `procs` host cores at once: thread p sweeps its
+    own lx x ly x (lz/procs) z-slab lattice (the share of one rank in a
+    z-decomposition), periodic in z instead of a halo exchange, so this is an
+    upper bound of a `procs`-rank CPU run.  The oracle's C calls release the
+    interpreter lock and keep no shared state.  The rate is that of the slowest
+    slab (all ranks would advance in step).  None unless lz/procs is even."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.oracle import OracleLattice
+    lx, ly, lz = shape
+    if lz % procs or (lz // procs) % 2:
+        return None
+    sub = (lx, ly, lz // procs)
+
+    def run(p):
+        O = OracleLattice(*sub)
+        O.set_couplings(*couplings())
+        O.hotstart(12345 + p, 0.5)
+        O.philox(12345 + p, 0)
+        d, sweeps, t0 = 0.4, 0, time.perf_counter()
+        while True:
+            O.agauge(1)
+            for _ in range(n_or):
+                O.link_or(1)
+            d, _ = O.ahiggs1(d, 1)
+            sweeps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                return sweeps, el
+    with ThreadPoolExecutor(max_workers=procs) as ex:
+        res = list(ex.map(run, range(procs)))
+    rate = min(k / el for k, el in res)
+    return {"value": lx * ly * lz * rate, "unit": "site-updates/s", "cores": procs, "kind": "port",
+            "sample": "%d threads, each sweeping (1 KP + %d OR + ahiggs1) its own %dx%dx%d z-slab "
+                      "of the %dx%dx%d workload for %.0f s (%d-%d sweeps) with oracle/lceq_oracle.c, "
+                      "periodic in z instead of a halo exchange: an upper bound of a %d-rank CPU "
+                      "run, at the slowest slab's rate"
+                      % (procs, n_or, *sub, *shape, seconds, min(k for k, _ in res),
+                         max(k for k, _ in res), procs)}
+
+
 REPLICA_SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 32), (40, 40, 40), (48, 48, 48),
                   (24, 24, 96)]
 
@@ -487,7 +528,7 @@ def main():
     ap.add_argument("--n-or", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-procs", type=int, default=16,
-                    help="host cores for the reference's multi-process baseline (refsched64)")
+                    help="host cores of the multi-core CPU baselines (z-slabs of the lattice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64"])
@@ -603,7 +644,11 @@ def main():
                                    "higgs_accept": hist[-1, 8], "delta": d},
     }
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds, args.n_or)
+        one = cpu_baseline(shape, args.cpu_seconds, args.n_or)
+        many = cpu_baseline_slabs(shape, args.cpu_seconds, args.n_or, args.cpu_procs)
+        rec["cpu_baseline"] = many if many is not None else one
+        if many is not None:
+            rec["config"]["cpu_port_one_thread"] = one
     print(json.dumps(rec), flush=True)
 
 
