@@ -10,6 +10,11 @@ Higgs-hopping Metropolis) + 1 Higgs Metropolis sweep (2 parities, device-tuned
 step), with plaquette / <rho> / <rho^2> / <a.F> accumulated every sweep.
 
     python bench.py [--steps K] [--warmup W] [--lattice 64,64,64] [--cpu-seconds S]
+                    [--cpu-procs P]
+
+cpu_baseline: the CPU restatement on P host cores (default 16), one z-slab
+of the lattice per thread (an upper bound of a P-rank CPU run); the
+single-threaded run of the whole lattice is config.cpu_port_one_thread.
 
 Before the timed region the autotuner picks the launch shape (threads per
 block, XCD-aware block order) from 10 sweeps per shape and restores the chain
