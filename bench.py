@@ -86,6 +86,12 @@ OR_TRAFFIC_64 = {512: 33.66e6 + 4.03e6, 256: 33.68e6 + 4.04e6, 128: 33.68e6 + 4.
                  64: 33.65e6 + 4.04e6}
 
 
+# couplings (trace convention) of the batched learned-vs-plain comparison:
+# the broken phase of the learned-proposal configuration, and one near the
+# symmetric/broken transition
+LEARNED_COUPLINGS = "0.34,-0.9;0.34,0.3"
+
+
 def couplings(beta_a_trace=0.33):
     from lceqcd_amd import couplings_from_trace_convention
     return couplings_from_trace_convention(9.0, beta_a_trace, -0.9, 0.1)
@@ -227,6 +233,62 @@ def replicas96(args):
                  "mean": float(tra2[:, r].mean()), "std": float(tra2[:, r].std())}
                 for r in range(nrep)]}, f)
     print(json.dumps(rec), flush=True)
+
+
+def _batch_history(B, nsweep, step_fn, chunk=1024):
+    """Run nsweep sweeps of a batch in chunks of at most `chunk` (the
+    history ring's size) and return the (nsweep, nrep, 16) history."""
+    import numpy as np
+    out, left = [], nsweep
+    while left > 0:
+        k = min(left, chunk)
+        step_fn(k)
+        out.append(B.history(k))
+        left -= k
+    return np.concatenate(out)
+
+
+def scan(args):
+    """<Tr A^2> against beta_2 at three beta_A (trace convention, beta_G = 9,
+    beta_4 = 0.1): one replica batch of 24^3 lattices, hot start, 1 KP + 4
+    OR + ahiggs1 per sweep; the transition is where <Tr A^2> falls fastest
+    and its variance per site (the susceptibility) peaks."""
+    import numpy as np
+    import lceqcd_amd
+    from lceqcd_amd.workloads import coupling
+    ba = [0.28, 0.34, 0.40]
+    b2 = np.linspace(-0.6, 2.5, 32)
+    grid = [(a, b) for a in ba for b in b2]
+    shape = tuple(int(x) for x in args.scan_lattice.split(","))
+    V = shape[0] * shape[1] * shape[2]
+    B = lceqcd_amd.Batch([shape] * len(grid))
+    B.set_couplings([coupling(a, b) for a, b in grid])
+    seeds = [20000 + r for r in range(len(grid))]
+    B.hotstart(seeds, 0.5)
+    B.rng_philox(seeds)
+    B.sweep(args.therm, n_or=4, delta=0.4, measflag=0)
+    B.sync()
+    t0 = time.perf_counter()
+    h = _batch_history(B, args.steps, lambda k: B.sweep(k, n_or=4, measflag=1))
+    el = time.perf_counter() - t0
+    tra2 = h[:, :, 2] / 2.0
+    res = {}
+    for i, a in enumerate(ba):
+        rows = []
+        for j, b in enumerate(b2):
+            x = tra2[:, i * len(b2) + j]
+            rows.append({"beta_2": float(b), "tr_a2": float(x.mean()),
+                         "chi": float(V * x.var()), "plaq": float(h[:, i * len(b2) + j, 0].mean())})
+        m = np.array([r["tr_a2"] for r in rows])
+        chi = np.array([r["chi"] for r in rows])
+        k = int(np.argmin(np.diff(m)))
+        res["%.2f" % a] = {"points": rows,
+                           "steepest_drop_between": [float(b2[k]), float(b2[k + 1])],
+                           "chi_peak_beta_2": float(b2[int(np.argmax(chi))])}
+    print(json.dumps({"workload": "scan", "lattice": list(shape), "replicas": len(grid),
+                      "therm": args.therm, "sweeps": args.steps, "seconds": el,
+                      "site_updates_per_s": len(grid) * V * args.steps / el,
+                      "beta_a": res}), flush=True)
 
 
 REF_SWEEP_BYTES_PER_SITE = 2 * 2 * HIGGS_PASS_BYTES_PER_SITE + 6 * LINK_PASS_BYTES_PER_SITE + 192
@@ -490,6 +552,145 @@ def learned(args, L0):
     print(json.dumps(rec), flush=True)
 
 
+def learned_batch(args, log=None):
+    """The learned Higgs proposal in batched form against the plain sweep
+    on the same replicas: R replicas of an L^3 lattice per coupling (two
+    couplings: the broken phase beta_A = 0.34, beta_2 = -0.9, and one near
+    the symmetric/broken transition), one network for all replicas (the
+    couplings are inputs), trained on records the batch's own plain sweeps
+    generate.  From one thermalised state per replica, S sweeps of the plain
+    batch (1 KP + 4 OR + ahiggs1) and S of the learned batch (1 KP + 4 OR +
+    learned update), both replayed from hipGraphs.  Per replica tau_int of
+    Tr A^2 (Madras-Sokal); effectively independent samples/s = sweeps/s /
+    (2 tau_int), summed over the replicas of a coupling; the amortised
+    figure charges the learned chain with the collection and training time."""
+    import numpy as np
+    import lceqcd_amd
+    from lceqcd_amd.analysis import tau_int
+    from lceqcd_amd.workloads import coupling
+    L0 = args.learned_lattice
+    shape = (L0,) * 3
+    V = L0 ** 3
+    groups = [tuple(float(v) for v in g.split(",")) for g in args.learned_couplings.split(";")]
+    R = args.learned_replicas
+    grid = [g for g in groups for _ in range(R)]
+    nrep = len(grid)
+    B = lceqcd_amd.Batch([shape] * nrep)
+    B.set_couplings([coupling(a, b) for a, b in grid])
+    seeds = [31000 + r for r in range(nrep)]
+    B.hotstart(seeds, 0.5)
+    B.rng_philox(seeds)
+    B.sweep(args.therm, n_or=4, delta=0.4, measflag=0)
+    M = lceqcd_amd.Mlp(seed=1)
+    stride = max(1, V // 2 // 3456)
+    B.sync()
+    t0 = time.perf_counter()
+    for c in range(args.learned_train_configs):
+        B.sweep(1, n_or=4, measflag=0, graph=False)
+        B.mlp_collect(M, nsteps=8, stride=stride)
+    t1 = time.perf_counter()
+    loss = M.train(args.train_steps, batch=8192, lr=2e-3, seed=1)
+    t2 = time.perf_counter()
+    if log:
+        log("learned batch: %d records collected in %.1f s, trained in %.1f s"
+            % (M.records(), t1 - t0, t2 - t1))
+    state = [(B.get_links(r), B.get_higgs(r)) for r in range(nrep)]
+    out = {}
+    for name in ("plain", "learned"):
+        for r in range(nrep):
+            B.put_links(r, state[r][0])
+            B.put_higgs(r, state[r][1])
+        B.rng_philox([41000 + 1000 * (name == "learned") + r for r in range(nrep)])
+        if name == "plain":
+            step = lambda k: B.sweep(k, n_or=4, measflag=1)
+        else:
+            step = lambda k: B.sweep_mlp(M, k, n_or=4, measflag=1)
+        step(8)  # graph capture outside the timed chain
+        B.history(8)
+        B.sync()
+        ts = time.perf_counter()
+        h = _batch_history(B, args.learned_sweeps, step)
+        el = time.perf_counter() - ts
+        out[name] = (h, el)
+        if log:
+            log("learned batch: %s chain %d sweeps in %.1f s" % (name, args.learned_sweeps, el))
+    res = {"lattice": list(shape), "replicas_per_coupling": R, "sweeps": args.learned_sweeps,
+           "train": {"configs": args.learned_train_configs, "records": M.records(),
+                     "steps": args.train_steps, "collect_s": t1 - t0, "train_s": t2 - t1,
+                     "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-20:].mean())},
+           "couplings": []}
+    for gi, (ba, b2) in enumerate(groups):
+        reps = range(gi * R, (gi + 1) * R)
+        g = {"beta_a": ba, "beta_2": b2}
+        for name in ("plain", "learned"):
+            h, el = out[name]
+            sps = args.learned_sweeps / el  # sweeps/s of every replica (all advance together)
+            taus, errs, acc, tra = [], [], [], []
+            for r in reps:
+                x = h[:, r, 2] / 2.0
+                t, dt, _ = tau_int(x)
+                taus.append(t)
+                errs.append(dt)
+                acc.append(float(h[:, r, 8].mean()))
+                tra.append(float(x.mean()))
+            indep = sum(sps / (2.0 * t) for t in taus)
+            g[name] = {"batch_sweeps_per_s": sps, "site_updates_per_s": sps * V * nrep,
+                       "tau_int": taus, "tau_int_err": errs, "acceptance": float(np.mean(acc)),
+                       "tr_a2_mean": float(np.mean(tra)), "indep_samples_per_s": indep}
+        g["learned_over_plain"] = g["learned"]["indep_samples_per_s"] / g["plain"]["indep_samples_per_s"]
+        # training charged to this coupling's share of the learned chain
+        share = R / nrep
+        el_l = out["learned"][1]
+        n_ind = g["learned"]["indep_samples_per_s"] * el_l
+        g["learned_over_plain_amortised"] = (n_ind / (el_l + share * (t2 - t0))) \
+            / g["plain"]["indep_samples_per_s"]
+        res["couplings"].append(g)
+    return res
+
+
+def hbm_resident_or(shape, n_or, tb_x, sweeps=10):
+    """The overrelaxation launch on a lattice too large for the 256 MB
+    Infinity Cache (160^3: 490 MB of fields, 557 MB moved per launch), timed
+    with HIP events as the 64^3 launch is, at the same launch shape."""
+    import lceqcd_amd
+    V = shape[0] * shape[1] * shape[2]
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(*couplings())
+    L.hotstart(12345, 0.5)
+    L.rng_philox(12345, 0)
+    L.set_shape(*tb_x)
+    d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 2, n_or=n_or, delta=0.4, measflag=0)
+    L.profile_begin(lceqcd_amd.KC_LINK_OR, 6 * n_or * sweeps)
+    L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, sweeps, n_or=n_or, delta=d, measflag=1)
+    ms, n = L.profile_end()
+    L.history(sweeps)
+    L.close()
+    us = ms * 1e3 / max(n, 1)
+    b = LINK_PASS_BYTES_PER_SITE * V
+    gbps = b / (us * 1e-6) / 1e9
+    return {"lattice": list(shape), "bytes_per_launch": b, "avg_launch_us": us, "launches": n,
+            "achieved": gbps, "frac": gbps / HBM_PEAK_GBPS}
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def flows(args):
+    """The plain-method, gauge-fixed and production configurations as runs
+    (lceqcd_amd.runs), one JSON line each."""
+    from lceqcd_amd import runs
+    from lceqcd_amd.workloads import C3
+    if args.workload == "plain":
+        rec = runs.run_plain()
+    elif args.workload == "gaugefix32":
+        rec = runs.run_gauge_fixed()
+    else:
+        rec = [runs.run_production(shape, ba, progress=_log) for shape in C3["shapes"]
+               for ba in C3["beta_a"]]
+    print(json.dumps({"workload": args.workload, "result": rec}), flush=True)
+
+
 def copy_rate(nbytes, reps):
     """Read+write GB/s of the library's plain device copy of nbytes."""
     import ctypes
@@ -536,7 +737,8 @@ def main():
                     help="host cores of the multi-core CPU baselines (z-slabs of the lattice)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
-                    choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64"])
+                    choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64", "scan",
+                             "mlpbatch", "plain", "gaugefix32", "production"])
     ap.add_argument("--sweeps", type=int, default=None,
                     help="sweeps per chain (learned workloads; default 60000 at 48^3, "
                          "20000 at 96^3: a few % statistical error on tau_int)")
@@ -545,6 +747,16 @@ def main():
     ap.add_argument("--therm", type=int, default=300,
                     help="thermalisation sweeps before training (learned workloads)")
     ap.add_argument("--hist-out", default=None)
+    ap.add_argument("--scan-lattice", default="24,24,24")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave the batched learned-vs-plain object out of the default line")
+    ap.add_argument("--learned-lattice", type=int, default=48)
+    ap.add_argument("--learned-replicas", type=int, default=4,
+                    help="replicas per coupling of the batched learned comparison")
+    ap.add_argument("--learned-couplings", default=LEARNED_COUPLINGS,
+                    help="beta_A,beta_2 pairs (trace convention) separated by ';'")
+    ap.add_argument("--learned-sweeps", type=int, default=20000)
+    ap.add_argument("--learned-train-configs", type=int, default=250)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="threads_per_block,xcd_order for the link and Higgs launches "
@@ -560,6 +772,14 @@ def main():
         return replicas96(args)
     if args.workload == "refsched64":
         return refsched64(args)
+    if args.workload == "scan":
+        return scan(args)
+    if args.workload == "mlpbatch":
+        print(json.dumps({"workload": "mlpbatch", "learned": learned_batch(args, log=_log)}),
+              flush=True)
+        return None
+    if args.workload in ("plain", "gaugefix32", "production"):
+        return flows(args)
     if args.sweeps is None:
         args.sweeps = 60000 if args.workload == "mlp48" else 20000
     if args.workload in ("mlp48", "mlp96"):
@@ -638,6 +858,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": (OR_TRAFFIC_64.get(L.get_shape()[0]) if shape == (64, 64, 64)
                         and L.get_shape()[1] else None),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this launch at this "
+                              "launch shape (profiles/k_link_or_64cubed_hbm_traffic.md), not "
+                              "measured in this run",
+            "residency": "the 64^3 field set (31 MB) sits in the 256 MB Infinity Cache, so "
+                         "frac is a fraction of the HBM peak, not of HBM traffic; the "
+                         "hbm_resident object times the same launch on a 160^3 lattice "
+                         "(490 MB) that no cache holds",
             "bytes_per_launch": launch_bytes,
             "avg_launch_us": avg_s * 1e6,
             "launches": or_n,
@@ -648,6 +875,13 @@ def main():
                                    "a04": hist[-1, 3], "a0kin": hist[-1, 1],
                                    "higgs_accept": hist[-1, 8], "delta": d},
     }
+    big = (160, 160, 160)
+    rec["roofline"]["hbm_resident"] = hbm_resident_or(big, args.n_or, L.get_shape())
+    rec["roofline"]["hbm_resident"]["frac_of_1GiB_copy"] = (
+        rec["roofline"]["hbm_resident"]["achieved"]
+        / rec["roofline"]["copy_ceiling"]["hbm_resident_1GiB_GBps"])
+    if not args.no_learned:
+        rec["learned"] = learned_batch(args, log=_log)
     if not args.no_cpu:
         one = cpu_baseline(shape, args.cpu_seconds, args.n_or)
         many = cpu_baseline_slabs(shape, args.cpu_seconds, args.n_or, args.cpu_procs)

@@ -209,8 +209,11 @@ int lceq_profile_end(lceq_lattice *L, double *total_ms, int *launches);
  * launch and every Higgs launch covers all replicas, and a sweep is one
  * replayed hipGraph.  No reference counterpart (the reference runs one
  * lattice per MPI job).  Replica r follows exactly the chain of a single
- * lattice with the same seed under lceq_sweep(LCEQ_SWEEP_KP_OR_METRO). */
+ * lattice with the same seed under lceq_sweep(LCEQ_SWEEP_KP_OR_METRO) (and,
+ * with lceq_batch_sweep_mlp, under lceq_sweep_mlp): a device pass counter
+ * numbers the Philox passes as the single lattice's pass counter does. */
 typedef struct lceq_batch lceq_batch;
+typedef struct lceq_mlp lceq_mlp;
 /* dims[3r..3r+2] = extents of replica r (even, >= 2) */
 int lceq_batch_create(int nrep, const int *dims, lceq_batch **out);
 int lceq_batch_destroy(lceq_batch *B);
@@ -248,6 +251,19 @@ int lceq_batch_get_shape(lceq_batch *B, int *threads_per_block, int *xcd_order);
  * NULL) receives 8 times per sweep, shapes (64,128,256,512) x (launch order,
  * XCD order). */
 int lceq_batch_autotune(lceq_batch *B, int n_or, int trials, double *ms);
+/* nsweep sweeps of 1 KP + n_or OR + the learned Higgs proposal of M for
+ * every replica, one network for all replicas (each replica's couplings are
+ * network inputs); history rows as lceq_batch_sweep, the Higgs observables
+ * always measured (as lceq_sweep_mlp).  The learned launches take 256-site
+ * blocks whatever the launch shape.  No reference counterpart; the plain
+ * update it replaces is f/ahiggs1.f:43-123.  Queued, no sync. */
+int lceq_batch_sweep_mlp(lceq_batch *B, lceq_mlp *M, int nsweep, int n_or, int measflag,
+                         int use_graph);
+/* training records of M from every replica at its own couplings, as
+ * lceq_mlp_collect: per parity the inputs at the current F, nsteps ahiggs1
+ * steps of that parity for every replica (kept in the chains; one Philox pass
+ * each, no history row), the new a in the same frame; every stride-th site */
+int lceq_batch_mlp_collect(lceq_batch *B, lceq_mlp *M, int nsteps, int stride);
 
 /* ---- learned Higgs proposal (no counterpart in the reference) -------------
  * Per site, a 10 -> 64 -> 64 -> 6 SiLU network (bf16 MFMA inference, fp32
@@ -260,7 +276,6 @@ int lceq_batch_autotune(lceq_batch *B, int n_or, int trials, double *ms);
  * fallback.  Training is maximum likelihood on moves of the plain update. */
 #define LCEQ_MLP_NPARAM 5254 /* W1[10][64] b1[64] W2[64][64] b2[64] W3[64][6] b3[6] */
 #define LCEQ_MLP_NSTATE 5274 /* parameters, feature mean[10], feature 1/scale[10] */
-typedef struct lceq_mlp lceq_mlp;
 /* Glorot-uniform weights from seed, zero biases, identity normalisation */
 int lceq_mlp_create(uint64_t seed, lceq_mlp **out);
 int lceq_mlp_destroy(lceq_mlp *M);
@@ -341,6 +356,13 @@ int lceq_rluxat(lceq_ranlux_state *R, int *lout, int *inout, int *k1, int *k2);
 /* the 16 integer seeds MPI rank `rank` derives in f/measinit.f:214-216:
  * RLUXGO(3, 314159 + rank, 0, 0), 16 RANLUX numbers, iseed = int(10^6 u) */
 int lceq_measinit_seeds(int rank, int *iseed);
+/* the device streams of L seeded as f/measinit.f:214-217 seeds the
+ * reference's random_number for rank `rank` (random_seed(put = iseed)): the
+ * Philox key is lceq_measinit_key(iseed) of the rank's 16 measinit seeds and
+ * the pass counter restarts at 0; iseed (or NULL) receives the 16 seeds */
+int lceq_rng_measinit(lceq_lattice *L, int rank, int *iseed);
+/* the Philox key of 16 measinit seeds: iseed[0] + 2^32 iseed[1] (each < 10^6) */
+uint64_t lceq_measinit_key(const int *iseed);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,8 @@ SIGNATURES = {
     "lceq_batch_set_shape": [VP, I, I],
     "lceq_batch_get_shape": [VP, IP, IP],
     "lceq_batch_autotune": [VP, I, I, DP],
+    "lceq_batch_sweep_mlp": [VP, VP, I, I, I, I],
+    "lceq_batch_mlp_collect": [VP, VP, I, I],
     "lceq_dims": [VP, IP],
     "lceq_mlp_create": [U64, C.POINTER(VP)],
     "lceq_mlp_destroy": [VP],
@@ -102,6 +104,7 @@ SIGNATURES = {
     "lceq_rluxin": [VP, IP],
     "lceq_rluxat": [VP, IP, IP, IP, IP],
     "lceq_measinit_seeds": [I, IP],
+    "lceq_rng_measinit": [VP, I, IP],
 }
 
 _lib = None
@@ -126,6 +129,8 @@ def load():
             f.restype = I
         lib.lceq_last_error.argtypes = []
         lib.lceq_last_error.restype = C.c_char_p
+        lib.lceq_measinit_key.argtypes = [IP]
+        lib.lceq_measinit_key.restype = U64
         _lib = lib
     return _lib
 

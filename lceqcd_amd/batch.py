@@ -84,6 +84,16 @@ class Batch:
         check(self._L.lceq_batch_sweep(self.h, nsweep, n_or, None if d is None else _dp(d),
                                        measflag, 1 if graph else 0))
 
+    def sweep_mlp(self, mlp, nsweep, n_or=4, measflag=1, graph=True):
+        """nsweep sweeps of 1 KP + n_or OR + the learned Higgs proposal of
+        ``mlp`` (a :class:`lceqcd_amd.Mlp`) for every replica."""
+        check(self._L.lceq_batch_sweep_mlp(self.h, mlp.h, int(nsweep), int(n_or), int(measflag),
+                                           1 if graph else 0))
+
+    def mlp_collect(self, mlp, nsteps=8, stride=1):
+        """Training records of ``mlp`` from every replica at its own couplings."""
+        check(self._L.lceq_batch_mlp_collect(self.h, mlp.h, int(nsteps), int(stride)))
+
     def sync(self):
         check(self._L.lceq_batch_sync(self.h))
 

@@ -53,12 +53,8 @@ extern "C" const char* lceq_last_error(void) { return g_err.c_str(); }
 extern "C" int lceq_create(int lx, int ly, int lz, lceq_lattice** out) {
   if (!out) return fail(LCEQ_ERR_ARG, "null output pointer");
   *out = nullptr;
-  if (lx < 2 || ly < 2 || lz < 2 || (lx & 1) || (ly & 1) || (lz & 1))
-    return fail(LCEQ_ERR_ARG, "extents must be even and >= 2 (got %d %d %d)", lx, ly, lz);
-  if ((int64_t)lx * ly * lz * 3 > INT32_MAX)
-    return fail(LCEQ_ERR_ARG, "lattice too large for 32-bit link counters");
-  if (ly >= (1 << 22) || lz >= (1 << 22))  // div_small's quotient range
-    return fail(LCEQ_ERR_ARG, "ly and lz must be below 2^22");
+  if (const char* why = geom_reject(lx, ly, lz))
+    return fail(LCEQ_ERR_ARG, "lattice %d x %d x %d: %s", lx, ly, lz, why);
   lceq_lattice* L = new lceq_lattice();
   memset(L, 0, sizeof(*L));
   L->g = geom_init(lx, ly, lz);

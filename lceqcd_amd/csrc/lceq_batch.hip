@@ -3,8 +3,8 @@
 // class and per Higgs parity covers every replica, so the device sees the sum
 // of all replicas' sites per launch.  Eight sweeps' launches are captured
 // once in a hipGraph and replayed; the Philox pass number comes from a
-// device-side sweep counter so that the captured graph stays valid from sweep
-// to sweep.  The launch shape (threads per block, XCD-aware block order) is a
+// device-side pass counter (with the sweep counter that indexes the history
+// rows) so that the captured graph stays valid from sweep to sweep.  The launch shape (threads per block, XCD-aware block order) is a
 // runtime choice that lceq_batch_autotune picks by timing.
 //
 // Replica r with seed s_r follows exactly the trajectory that a single
@@ -20,6 +20,7 @@
 
 #include "../../include/lceq.h"
 #include "lceq_internal.hpp"
+#include "lceq_mlp.hpp"
 #include "lceq_sites.hpp"
 
 using namespace lceq;
@@ -35,6 +36,7 @@ struct Replica {
   double beta, beta_a, beta_2, beta_4;
   Philox key;
   int blk0, nb;     // first global block, number of blocks
+  int mblk0, mnb;   // the same for the learned Higgs launches (256 sites per block)
   double* part;     // [LCEQ_NSLOT][LCEQ_NQ][pstride]
   int pstride;
   double* obs;      // [LCEQ_OBS_N]
@@ -44,8 +46,10 @@ struct BatchCommon {
   const Replica* reps;
   const int* blk2rep;
   int total_nb;
-  const uint64_t* sweep;  // device sweep counter
-  int passes_per_sweep;
+  // device counters: ctr[0] sweeps (history row), ctr[1] the Philox pass
+  // of the first launch of the current sweep; launch j of a sweep uses pass
+  // ctr[1] + j, as the j-th next_pass() of a single lattice's sweep
+  const uint64_t* ctr;
   int xcd;                // XCD-aware block order (else launch order)
 };
 
@@ -76,7 +80,7 @@ __global__ __launch_bounds__(TBX) void k_link_batch(BatchCommon c, int eo, int m
     p.beta_a = R->beta_a;
     p.fed = nullptr;
     p.key = R->key;
-    p.pass = (*c.sweep) * (uint64_t)c.passes_per_sweep + (uint64_t)pass_in_sweep;
+    p.pass = c.ctr[1] + (uint64_t)pass_in_sweep;
     p.measflag = measflag;
     p.renorm = 1;
     link_update_site<KP, false>(p, s, red);
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(TBX) void k_higgs_batch(BatchCommon c, int eo, int 
     p.delta = R->obs + OBS_DELTA;
     p.fed = nullptr;
     p.key = R->key;
-    p.pass = (*c.sweep) * (uint64_t)c.passes_per_sweep + (uint64_t)pass_in_sweep;
+    p.pass = c.ctr[1] + (uint64_t)pass_in_sweep;
     p.measflag = measflag;
     higgs_update_site<1, false>(p, s, red);
   }
@@ -115,11 +119,41 @@ __global__ __launch_bounds__(TBX) void k_higgs_batch(BatchCommon c, int eo, int 
                                lblk);
 }
 
+// One Higgs parity of every replica with the learned proposal: the per-site
+// code of the single-lattice launch (mlp::mlp_higgs_site, 64 sites per wave,
+// 256 per block) on the replica's fields, couplings, key and partial-sum
+// slots, one network for all replicas (the couplings are network inputs).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_higgs_mlp_batch(
+    BatchCommon c, const int* __restrict__ mblk2rep, int total_mnb,
+    const mlp::MlpPack* __restrict__ pack, int eo, int pass_in_sweep) {
+  __shared__ mlp::MlpLds lds[4];
+  int gblk = c.xcd ? xcd_block(total_mnb) : (int)blockIdx.x;
+  const Replica* R = c.reps + mblk2rep[gblk];
+  mlp::MlpArgs a;
+  a.g = R->g;
+  a.U = R->U;
+  a.A = R->A;
+  a.eo = eo;
+  a.beta_a = R->beta_a;
+  a.beta_2 = R->beta_2;
+  a.beta_4 = R->beta_4;
+  a.key = R->key;
+  a.pass = c.ctr[1] + (uint64_t)pass_in_sweep;
+  a.pack = pack;
+  a.part = R->part + (size_t)(6 + eo) * LCEQ_NQ * R->pstride;
+  a.pstride = R->pstride;
+  a.nb = R->mnb;
+  mlp::mlp_higgs_site<4>(a, gblk - R->mblk0, lds[threadIdx.x >> 6]);
+}
+
 // Per-replica close of a sweep (block r): the same reduction and slot order
 // as k_finalize of the single-lattice path, then the history row.
+// higgs_kind 1: ahiggs1 (Higgs observables with meas, step adaptation);
+// 2: the learned proposal (Higgs observables always, as lceq_sweep_mlp
+// records them, no step adaptation; its partials come from R->mnb blocks).
 __global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restrict__ reps,
-                                                         int link_kind, int meas,
-                                                         const uint64_t* __restrict__ sweep,
+                                                         int link_kind, int meas, int higgs_kind,
+                                                         const uint64_t* __restrict__ ctr,
                                                          double* __restrict__ hist, int hist_cap,
                                                          int nrep) {
   constexpr int NV = 6 * 3 + 2 * 4;
@@ -136,10 +170,11 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restri
       size_t row = q < 18 ? (size_t)(q / 3) * LCEQ_NQ + q % 3
                           : (size_t)(6 + (q - 18) / 4) * LCEQ_NQ + (q - 18) % 4;
       const double* x = R->part + row * R->pstride;
-      for (int i0 = lane; i0 < R->nb; i0 += 64 * 8) {
+      int nbq = q < 18 || higgs_kind == 1 ? R->nb : R->mnb;
+      for (int i0 = lane; i0 < nbq; i0 += 64 * 8) {
         double y[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < R->nb ? x[i0 + 64 * u] : 0.0;
+        for (int u = 0; u < 8; ++u) y[u] = i0 + 64 * u < nbq ? x[i0 + 64 * u] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u) t += y[u];
       }
@@ -174,24 +209,32 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restri
   }
   double acc = v[18] + v[22];
   double r2 = v[19] + v[23], r4 = v[20] + v[24], kin = v[21] + v[25];
-  if (meas == 1) {
+  int hmeas = higgs_kind == 2 ? 1 : meas;
+  if (hmeas == 1) {
     obs[OBS_A02] = r2 / vol;
     obs[OBS_A04] = r4 / vol;
     obs[OBS_A0KIN] = kin / (6.0 * vol);
   }
   double accept = acc / vol;
   obs[OBS_ACC_HIGGS] = accept;
-  double d = obs[OBS_DELTA];
-  if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
-  obs[OBS_DELTA] = d;
+  if (higgs_kind == 1) {
+    double d = obs[OBS_DELTA];
+    if (fabs(accept - 0.5) > 0.1) d = (accept > 0.5) ? 1.1131241 * d : 0.8989669 * d;
+    obs[OBS_DELTA] = d;
+  }
   if (meas == 1 && hist) {
-    double* row = hist + ((size_t)(*sweep % (uint64_t)hist_cap) * nrep + blockIdx.x) * LCEQ_OBS_N;
+    double* row = hist + ((size_t)(ctr[0] % (uint64_t)hist_cap) * nrep + blockIdx.x) * LCEQ_OBS_N;
     for (int i = 0; i < LCEQ_OBS_N; ++i) row[i] = obs[i];
   }
 }
 
-__global__ void k_tick(uint64_t* sweep) {
-  if (threadIdx.x == 0) *sweep += 1;
+// advance the counters past a sweep (dsweep = 1) or past collect steps
+// (dsweep = 0) that used dpass passes
+__global__ void k_tick(uint64_t* ctr, int dsweep, int dpass) {
+  if (threadIdx.x == 0) {
+    ctr[0] += (uint64_t)dsweep;
+    ctr[1] += (uint64_t)dpass;
+  }
 }
 
 }  // namespace
@@ -219,12 +262,16 @@ struct lceq_batch {
   double *pool, *parts, *obs, *hist;
   size_t pool_doubles, parts_doubles;
   int hist_cap;
-  uint64_t* d_sweep;
+  uint64_t* d_ctr;       // [2]: sweeps, Philox pass base (BatchCommon::ctr)
   uint64_t sweeps_done;  // host mirror of the device counter
   hipStream_t stream;
   hipGraphExec_t graph;  // GRAPH_SWEEPS sweeps
   int graph_n_or, graph_meas;
+  const mlp::MlpPack* graph_pack;  // the learned proposal's weights, or null (ahiggs1)
   int tb, xcd;           // launch shape
+  // learned Higgs launches: 256-site blocks, independent of the launch shape
+  int* d_mblk2rep;
+  int total_mnb;
 };
 
 constexpr int GRAPH_SWEEPS = 8;
@@ -271,16 +318,12 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   B->field_off.resize(nrep);
   int64_t pool = 0, parts = 0;
   int nb_total = 0;
-  std::vector<int> blk2rep;
+  std::vector<int> blk2rep, mblk2rep;
   for (int r = 0; r < nrep; ++r) {
     int lx = dims[3 * r], ly = dims[3 * r + 1], lz = dims[3 * r + 2];
-    if (lx < 2 || ly < 2 || lz < 2 || (lx & 1) || (ly & 1) || (lz & 1)) {
+    if (const char* why = geom_reject(lx, ly, lz)) {
       delete B;
-      return bfail(LCEQ_ERR_ARG, "replica %d: extents must be even and >= 2", r);
-    }
-    if ((int64_t)lx * ly * lz * 3 > INT32_MAX || ly >= (1 << 22) || lz >= (1 << 22)) {
-      delete B;
-      return bfail(LCEQ_ERR_ARG, "replica %d: lattice too large", r);
+      return bfail(LCEQ_ERR_ARG, "replica %d (%d x %d x %d): %s", r, lx, ly, lz, why);
     }
     Replica& R = B->reps[r];
     memset(&R, 0, sizeof(R));
@@ -294,9 +337,13 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
     parts += (int64_t)LCEQ_NSLOT * LCEQ_NQ * R.pstride;
     for (int k = 0; k < R.nb; ++k) blk2rep.push_back(r);
     nb_total += R.nb;
+    R.mnb = (R.g.n + mlp::MLP_TB - 1) / mlp::MLP_TB;
+    R.mblk0 = (int)mblk2rep.size();
+    for (int k = 0; k < R.mnb; ++k) mblk2rep.push_back(r);
     B->total_sites += 2 * (int64_t)R.g.n;
   }
   B->total_nb = nb_total;
+  B->total_mnb = (int)mblk2rep.size();
   B->pool_doubles = pool;
   B->parts_doubles = parts;
   B->hist_cap = 1024;
@@ -313,7 +360,8 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   alloc((void**)&B->hist, (size_t)B->hist_cap * nrep * LCEQ_OBS_N * sizeof(double));
   alloc((void**)&B->d_reps, nrep * sizeof(Replica));
   alloc((void**)&B->d_blk2rep, blk2rep.size() * sizeof(int));
-  alloc((void**)&B->d_sweep, sizeof(uint64_t));
+  alloc((void**)&B->d_mblk2rep, mblk2rep.size() * sizeof(int));
+  alloc((void**)&B->d_ctr, 2 * sizeof(uint64_t));
   if (rc) {
     lceq_batch_destroy(B);
     return rc;
@@ -334,7 +382,9 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   for (int r = 0; r < nrep; ++r) obs0[(size_t)r * LCEQ_OBS_N + OBS_DELTA] = 0.4;
   (void)hipMemcpyAsync(B->obs, obs0.data(), obs0.size() * sizeof(double), hipMemcpyHostToDevice,
                        B->stream);
-  (void)hipMemsetAsync(B->d_sweep, 0, sizeof(uint64_t), B->stream);
+  (void)hipMemsetAsync(B->d_ctr, 0, 2 * sizeof(uint64_t), B->stream);
+  (void)hipMemcpyAsync(B->d_mblk2rep, mblk2rep.data(), mblk2rep.size() * sizeof(int),
+                       hipMemcpyHostToDevice, B->stream);
   if (batch_layout(B, 256, 1) != LCEQ_OK) {
     lceq_batch_destroy(B);
     return bfail(LCEQ_ERR_HIP, "batch initialisation failed");
@@ -347,7 +397,8 @@ extern "C" int lceq_batch_destroy(lceq_batch* B) {
   if (!B) return LCEQ_OK;
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
-  void* bufs[] = {B->pool, B->parts, B->obs, B->hist, B->d_reps, B->d_blk2rep, B->d_sweep};
+  void* bufs[] = {B->pool, B->parts, B->obs, B->hist, B->d_reps, B->d_blk2rep, B->d_mblk2rep,
+                  B->d_ctr};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -377,8 +428,8 @@ extern "C" int lceq_batch_rng_philox(lceq_batch* B, const uint64_t* seeds) {
   if (!B || !seeds) return bfail(LCEQ_ERR_ARG, "null argument");
   for (int r = 0; r < B->nrep; ++r)
     B->reps[r].key = Philox{(uint32_t)seeds[r], (uint32_t)(seeds[r] >> 32)};
-  uint64_t zero = 0;
-  BHIP(hipMemcpyAsync(B->d_sweep, &zero, sizeof(zero), hipMemcpyHostToDevice, B->stream));
+  uint64_t zero[2] = {0, 0};
+  BHIP(hipMemcpyAsync(B->d_ctr, zero, sizeof(zero), hipMemcpyHostToDevice, B->stream));
   B->sweeps_done = 0;
   return batch_upload_table(B);
 }
@@ -441,7 +492,7 @@ extern "C" int lceq_batch_get_higgs(lceq_batch* B, int r, double* a, int64_t ld)
 }
 
 template <int TBX>
-static void enqueue_updates(lceq_batch* B, const BatchCommon& c, int n_or, int measflag) {
+static void enqueue_links(lceq_batch* B, const BatchCommon& c, int n_or, int measflag) {
   dim3 grid(B->total_nb), blk(TBX);
   int pass = 0;
   for (int k = -1; k < n_or; ++k) {
@@ -458,26 +509,86 @@ static void enqueue_updates(lceq_batch* B, const BatchCommon& c, int n_or, int m
         ++pass;
       }
   }
-  for (int eo = 0; eo < 2; ++eo) {
-    hipLaunchKernelGGL((k_higgs_batch<TBX>), grid, blk, 0, B->stream, c, eo, pass, measflag);
-    ++pass;
-  }
+}
+
+// ahiggs1 of parity eo for every replica at pass ctr[1] + pass_in_sweep
+template <int TBX>
+static void enqueue_higgs(lceq_batch* B, const BatchCommon& c, int eo, int pass_in_sweep,
+                          int measflag) {
+  hipLaunchKernelGGL((k_higgs_batch<TBX>), dim3(B->total_nb), dim3(TBX), 0, B->stream, c, eo,
+                     pass_in_sweep, measflag);
+}
+
+static BatchCommon batch_common(const lceq_batch* B) {
+  return BatchCommon{B->d_reps, B->d_blk2rep, B->total_nb, B->d_ctr, B->xcd};
 }
 
 // Enqueue one sweep of every replica: 1 KP + n_or overrelaxation link
-// sweeps + ahiggs1, then the per-replica finalize and the counter tick.
-static void enqueue_sweep(lceq_batch* B, int n_or, int measflag) {
-  BatchCommon c{B->d_reps, B->d_blk2rep, B->total_nb, B->d_sweep, 6 * (1 + n_or) + 2, B->xcd};
+// sweeps + the Higgs update (ahiggs1, or the learned proposal of `pack`),
+// then the per-replica finalize and the counter tick.  Passes as a single
+// lattice's sweep: 6 per link sweep, then one per Higgs parity.
+static void enqueue_sweep(lceq_batch* B, int n_or, int measflag,
+                          const mlp::MlpPack* pack = nullptr) {
+  BatchCommon c = batch_common(B);
+  int hp = 6 * (1 + n_or);  // pass of the first Higgs parity
   switch (B->tb) {
-    case 64: enqueue_updates<64>(B, c, n_or, measflag); break;
-    case 128: enqueue_updates<128>(B, c, n_or, measflag); break;
-    case 512: enqueue_updates<512>(B, c, n_or, measflag); break;
-    default: enqueue_updates<256>(B, c, n_or, measflag); break;
+    case 64: enqueue_links<64>(B, c, n_or, measflag); break;
+    case 128: enqueue_links<128>(B, c, n_or, measflag); break;
+    case 512: enqueue_links<512>(B, c, n_or, measflag); break;
+    default: enqueue_links<256>(B, c, n_or, measflag); break;
+  }
+  for (int eo = 0; eo < 2; ++eo) {
+    if (pack) {
+      hipLaunchKernelGGL(k_higgs_mlp_batch, dim3(B->total_mnb), dim3(mlp::MLP_TB), 0, B->stream,
+                         c, B->d_mblk2rep, B->total_mnb, pack, eo, hp + eo);
+      continue;
+    }
+    switch (B->tb) {
+      case 64: enqueue_higgs<64>(B, c, eo, hp + eo, measflag); break;
+      case 128: enqueue_higgs<128>(B, c, eo, hp + eo, measflag); break;
+      case 512: enqueue_higgs<512>(B, c, eo, hp + eo, measflag); break;
+      default: enqueue_higgs<256>(B, c, eo, hp + eo, measflag); break;
+    }
   }
   int link_kind = measflag ? (n_or ? 2 : 1) : 0;
   hipLaunchKernelGGL(k_finalize_batch, dim3(B->nrep), dim3(FIN_TB), 0, B->stream, B->d_reps,
-                     link_kind, measflag, B->d_sweep, B->hist, B->hist_cap, B->nrep);
-  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, B->stream, B->d_sweep);
+                     link_kind, measflag, pack ? 2 : 1, B->d_ctr, B->hist, B->hist_cap, B->nrep);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, B->stream, B->d_ctr, 1, hp + 2);
+}
+
+// nsweep sweeps, GRAPH_SWEEPS at a time from a captured graph (cached per
+// schedule: n_or, measflag and the Higgs update), the rest launched directly
+static int batch_run(lceq_batch* B, int nsweep, int n_or, int measflag, int use_graph,
+                     const mlp::MlpPack* pack) {
+  int direct = nsweep;
+  if (use_graph && nsweep >= GRAPH_SWEEPS) {
+    if (!B->graph || B->graph_n_or != n_or || B->graph_meas != measflag ||
+        B->graph_pack != pack) {
+      if (B->graph) {
+        BHIP(hipGraphExecDestroy(B->graph));
+        B->graph = nullptr;
+      }
+      hipGraph_t g;
+      BHIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < GRAPH_SWEEPS; ++k) enqueue_sweep(B, n_or, measflag, pack);
+      BHIP(hipStreamEndCapture(B->stream, &g));
+      hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        B->graph = nullptr;
+        return bfail(LCEQ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      }
+      B->graph_n_or = n_or;
+      B->graph_meas = measflag;
+      B->graph_pack = pack;
+    }
+    for (int it = 0; it < nsweep / GRAPH_SWEEPS; ++it) BHIP(hipGraphLaunch(B->graph, B->stream));
+    direct = nsweep % GRAPH_SWEEPS;
+  }
+  for (int it = 0; it < direct; ++it) enqueue_sweep(B, n_or, measflag, pack);
+  BHIP(hipGetLastError());
+  B->sweeps_done += nsweep;
+  return LCEQ_OK;
 }
 
 extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const double* delta,
@@ -489,32 +600,51 @@ extern "C" int lceq_batch_sweep(lceq_batch* B, int nsweep, int n_or, const doubl
                           hipMemcpyHostToDevice, B->stream));
     BHIP(hipStreamSynchronize(B->stream));
   }
-  int direct = nsweep;
-  if (use_graph && nsweep >= GRAPH_SWEEPS) {
-    if (!B->graph || B->graph_n_or != n_or || B->graph_meas != measflag) {
-      if (B->graph) {
-        BHIP(hipGraphExecDestroy(B->graph));
-        B->graph = nullptr;
+  return batch_run(B, nsweep, n_or, measflag, use_graph, nullptr);
+}
+
+extern "C" int lceq_batch_sweep_mlp(lceq_batch* B, lceq_mlp* M, int nsweep, int n_or,
+                                    int measflag, int use_graph) {
+  if (!B || !M || nsweep < 0 || n_or < 0) return bfail(LCEQ_ERR_ARG, "bad sweep arguments");
+  return batch_run(B, nsweep, n_or, measflag, use_graph, mlp::device_pack(M));
+}
+
+// Training records from every replica at its own couplings, as
+// lceq_mlp_collect does for one lattice: per parity, the inputs at the
+// current F, nsteps ahiggs1 steps of that parity for every replica (kept in
+// the chains; each takes one Philox pass, no history row), the new a.
+extern "C" int lceq_batch_mlp_collect(lceq_batch* B, lceq_mlp* M, int nsteps, int stride) {
+  if (!B || !M) return bfail(LCEQ_ERR_ARG, "null argument");
+  if (nsteps < 1 || stride < 1) return bfail(LCEQ_ERR_ARG, "nsteps and stride must be >= 1");
+  std::vector<int64_t> off(B->nrep + 1, 0);
+  for (int r = 0; r < B->nrep; ++r) off[r + 1] = off[r] + (B->reps[r].g.n + stride - 1) / stride;
+  const int64_t nrec = off[B->nrep];
+  float* base = mlp::reserve_records(M, 2 * nrec);
+  if (!base) return LCEQ_ERR_HIP;
+  BatchCommon c = batch_common(B);
+  for (int eo = 0; eo < 2; ++eo) {
+    for (int target = 0; target < 2; ++target) {
+      for (int r = 0; r < B->nrep; ++r) {
+        const Replica& R = B->reps[r];
+        mlp::launch_record(B->stream, R.g, R.U, R.A, eo, stride, (int)(off[r + 1] - off[r]),
+                           R.beta_a, R.beta_2, R.beta_4,
+                           base + (size_t)(eo * nrec + off[r]) * mlp::NREC, target);
       }
-      hipGraph_t g;
-      BHIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < GRAPH_SWEEPS; ++k) enqueue_sweep(B, n_or, measflag);
-      BHIP(hipStreamEndCapture(B->stream, &g));
-      hipError_t e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        B->graph = nullptr;
-        return bfail(LCEQ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      if (target) break;
+      for (int h = 0; h < nsteps; ++h) {
+        switch (B->tb) {
+          case 64: enqueue_higgs<64>(B, c, eo, 0, 0); break;
+          case 128: enqueue_higgs<128>(B, c, eo, 0, 0); break;
+          case 512: enqueue_higgs<512>(B, c, eo, 0, 0); break;
+          default: enqueue_higgs<256>(B, c, eo, 0, 0); break;
+        }
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, B->stream, B->d_ctr, 0, 1);
       }
-      B->graph_n_or = n_or;
-      B->graph_meas = measflag;
     }
-    for (int it = 0; it < nsweep / GRAPH_SWEEPS; ++it) BHIP(hipGraphLaunch(B->graph, B->stream));
-    direct = nsweep % GRAPH_SWEEPS;
   }
-  for (int it = 0; it < direct; ++it) enqueue_sweep(B, n_or, measflag);
   BHIP(hipGetLastError());
-  B->sweeps_done += nsweep;
+  BHIP(hipStreamSynchronize(B->stream));
+  mlp::commit_records(M, 2 * nrec);
   return LCEQ_OK;
 }
 
@@ -577,11 +707,11 @@ extern "C" int lceq_batch_autotune(lceq_batch* B, int n_or, int trials, double* 
   size_t pbytes = B->pool_doubles * sizeof(double), obytes = (size_t)B->nrep * LCEQ_OBS_N * sizeof(double);
   BHIP(hipMalloc((void**)&pool0, pbytes));
   int rc = LCEQ_OK;
-  if (hipMalloc((void**)&obs0, obytes) != hipSuccess || hipMalloc((void**)&sw0, 8) != hipSuccess)
+  if (hipMalloc((void**)&obs0, obytes) != hipSuccess || hipMalloc((void**)&sw0, 16) != hipSuccess)
     rc = bfail(LCEQ_ERR_HIP, "hipMalloc failed");
   if (!rc && (hipMemcpyAsync(pool0, B->pool, pbytes, hipMemcpyDeviceToDevice, B->stream) ||
               hipMemcpyAsync(obs0, B->obs, obytes, hipMemcpyDeviceToDevice, B->stream) ||
-              hipMemcpyAsync(sw0, B->d_sweep, 8, hipMemcpyDeviceToDevice, B->stream)))
+              hipMemcpyAsync(sw0, B->d_ctr, 16, hipMemcpyDeviceToDevice, B->stream)))
     rc = bfail(LCEQ_ERR_HIP, "snapshot failed");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) || hipEventCreate(&e1))) rc = bfail(LCEQ_ERR_HIP, "events");
@@ -610,7 +740,7 @@ extern "C" int lceq_batch_autotune(lceq_batch* B, int n_or, int trials, double* 
   }
   if (!rc && (hipMemcpyAsync(B->pool, pool0, pbytes, hipMemcpyDeviceToDevice, B->stream) ||
               hipMemcpyAsync(B->obs, obs0, obytes, hipMemcpyDeviceToDevice, B->stream) ||
-              hipMemcpyAsync(B->d_sweep, sw0, 8, hipMemcpyDeviceToDevice, B->stream) ||
+              hipMemcpyAsync(B->d_ctr, sw0, 16, hipMemcpyDeviceToDevice, B->stream) ||
               hipStreamSynchronize(B->stream)))
     rc = bfail(LCEQ_ERR_HIP, "restore failed");
   if (e0) (void)hipEventDestroy(e0);

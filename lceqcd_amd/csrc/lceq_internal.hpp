@@ -70,6 +70,22 @@ namespace lceq {
 // record a failure for lceq_last_error() and return `code`
 int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
+// Why an lx x ly x lz lattice cannot be created, or nullptr if it can.
+// Besides even extents >= 2: 3 V links fit the 32-bit link counters, the
+// index divisions need ly, lz < 2^22, and the device loads address a site
+// of a class array as an unsigned 32-bit byte offset s * 16 (16-byte link
+// halves), so a parity holds at most 2^28 sites (N <= 2^28 keeps
+// (npad - 1) * 16 below 2^32).  Checked before any device call.
+inline const char* geom_reject(int lx, int ly, int lz) {
+  if (lx < 2 || ly < 2 || lz < 2 || (lx & 1) || (ly & 1) || (lz & 1))
+    return "extents must be even and >= 2";
+  if ((int64_t)lx * ly * lz * 3 > INT32_MAX) return "lattice too large for 32-bit link counters";
+  if (ly >= (1 << 22) || lz >= (1 << 22)) return "ly and lz must be below 2^22";
+  if ((int64_t)lx * ly * lz / 2 > ((int64_t)1 << 28))
+    return "more than 2^28 sites per parity (32-bit byte offsets of 16-byte link halves)";
+  return nullptr;
+}
+
 enum KernelClass { KC_NONE = 0, KC_LINK_KP = 1, KC_LINK_OR = 2, KC_HIGGS = 3, KC_GFIX = 4 };
 
 int blocks_for(int n);

@@ -227,3 +227,24 @@ extern "C" int lceq_measinit_seeds(int rank, int* iseed) {
   for (int i = 0; i < 16; ++i) iseed[i] = (int)(1000000.0f * rvec[i]);
   return LCEQ_OK;
 }
+
+// The device streams of lattice L seeded from rank `rank`'s measinit seeds,
+// as f/measinit.f:214-217 seeds the reference's random_number with them
+// (random_seed(put = iseed)): the Philox key is iseed(1) + 2^32 iseed(2)
+// (each below 10^6), the pass counter restarts at 0.  iseed (or NULL)
+// receives the 16 seeds, which the reference leaves in COMMON /paramet/.
+extern "C" int lceq_rng_measinit(lceq_lattice* L, int rank, int* iseed) {
+  if (!L) return lceq::set_error(LCEQ_ERR_ARG, "null lattice handle");
+  int s[16];
+  int rc = lceq_measinit_seeds(rank, s);
+  if (rc) return rc;
+  rc = lceq_rng_philox(L, lceq_measinit_key(s), 0);
+  if (rc) return rc;
+  if (iseed) memcpy(iseed, s, sizeof s);
+  return LCEQ_OK;
+}
+
+extern "C" uint64_t lceq_measinit_key(const int* iseed) {
+  if (!iseed) return 0;
+  return (uint64_t)(uint32_t)iseed[0] | ((uint64_t)(uint32_t)iseed[1] << 32);
+}

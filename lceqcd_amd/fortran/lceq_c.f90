@@ -56,6 +56,16 @@ module lceq_c
       type(c_ptr), value :: h
       integer(c_int64_t), value :: seed, pass
     end function
+    integer(c_int) function lceq_rng_measinit(h, rank, iseed) bind(C, name="lceq_rng_measinit")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: rank
+      integer(c_int) :: iseed(16)
+    end function
+    integer(c_int64_t) function lceq_measinit_key(iseed) bind(C, name="lceq_measinit_key")
+      import :: c_int, c_int64_t
+      integer(c_int) :: iseed(16)
+    end function
     integer(c_int) function lceq_rng_feed(h, uniforms, n) bind(C, name="lceq_rng_feed")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h

@@ -16,7 +16,10 @@
 *     random_number intrinsic -- the reference's own stream, seeded as the
 *     reference seeds it -- and feeds them to the device, which consumes
 *     them in the reference's order.  With LCEQ_DROPIN_RNG=philox the
-*     device's counter-based streams (seed iseed(1)) are used instead.
+*     device's counter-based streams are used instead, seeded as
+*     f/measinit.f:214-217 seeds random_number on rank 0 (lceq_rng_measinit:
+*     key iseed(1) + 2^32 iseed(2) of the RANLUX seeds, which are also left
+*     in COMMON iseed as the reference's measinit leaves them).
 *
 *     Coherence: the first call uploads u and a from COMMON /linkvar/ to
 *     the device lattice.  Every routine that changes a field copies it
@@ -33,6 +36,9 @@
       INCLUDE 'input_parameter.inc'
       INCLUDE 'gaugefield.inc'
       integer(c_int) rc
+      integer(c_int) ls(16)
+      character(len=16) mode
+      integer ln, st
       if (c_associated(lceq_handle)) return
       if (nproc .ne. 1) then
         write(*,*) 'lceq_dropin: one rank only, nproc =', nproc
@@ -42,8 +48,12 @@
      &                 lceq_handle)
       call lceq_check(rc, 'lceq_create')
       call lceq_dropin_params
-      call lceq_check(lceq_rng_philox(lceq_handle,
-     &     int(iseed(1), c_int64_t), 0_c_int64_t), 'rng_philox')
+      call get_environment_variable('LCEQ_DROPIN_RNG', mode, ln, st)
+      if (st .eq. 0 .and. mode(1:6) .eq. 'philox') then
+        call lceq_check(lceq_rng_measinit(lceq_handle, 0, ls),
+     &       'rng_measinit')
+        iseed(1:16) = ls
+      endif
       call lceq_dropin_upload
       end
 

@@ -9,7 +9,7 @@
 ! reference's names (f/measinit.f:246-273) and record formats (f/meas1.f:91,
 ! f/correlator.f:168-181, f/meas2a.f:83-97, f/meas1.f:70-76).
 !
-!   lceq_main [inputfile] [seed] [ntherm] [realbytes] [corrflag]
+!   lceq_main [inputfile] [seed | ranlux] [ntherm] [realbytes] [corrflag]
 !
 ! ntherm (default 100, as f/main.f:20) is the number of thermalisation sweeps;
 ! realbytes (8, or 4 for files of the as-shipped single-precision build) is
@@ -20,8 +20,12 @@
 !
 ! Differences from the reference, by design: one device holds the whole
 ! lattice (dimproc and nproc are read and ignored); the uniforms come from the
-! counter-based Philox streams keyed by `seed` (default 314159, the base of the
-! reference's RANLUX seeds, f/measinit.f:214-217); init = 0 or 1 is the hot
+! counter-based Philox streams, seeded by default (or with the argument
+! `ranlux`) as f/measinit.f:214-217 seeds the reference's random_number on rank
+! 0: RANLUX level 3 from 314159, 16 seeds int(10^6 u), which also fill
+! iseed(1:16) in the joblog; the streams' key is iseed(1) + 2^32 iseed(2)
+! (lceq_rng_measinit).  A number as `seed` keys the streams directly.  The hot
+! start draws from the same key.  init = 0 or 1 is the hot
 ! start (Haar links, Higgs N(0, 0.5^2)), 2 is unitconf, 4 is testconf, 3 is
 ! oldconf (f/confget.f:318-526) from the file f/measinit.f:246-250 names, or,
 ! when that lower-case name does not exist, from the upper-case name that
@@ -42,6 +46,8 @@ program lceq_main
   character(len=49) :: polyak_file, corra2_file
   integer :: betaint, xint, yint, isweep, meas, mstart, nmeas, iter1, iter_or, z, ntherm
   integer(c_int64_t) :: seed, nbad
+  integer(c_int) :: iseed(16)
+  logical :: ranlux_seed
   integer :: realbytes, corr_flag
   real(8) :: pmean
   real(8), allocatable :: pcorr(:)
@@ -53,10 +59,15 @@ program lceq_main
 
   fin = 'inputfile'
   if (command_argument_count() >= 1) call get_command_argument(1, fin)
-  seed = 314159
+  ranlux_seed = .true.
+  iseed = 0
+  seed = 0
   if (command_argument_count() >= 2) then
     call get_command_argument(2, arg)
-    read(arg, *) seed
+    if (trim(arg) /= 'ranlux') then
+      read(arg, *) seed
+      ranlux_seed = .false.
+    end if
   end if
   ntherm = 100
   if (command_argument_count() >= 3) then
@@ -118,7 +129,13 @@ program lceq_main
   call lceq_check(lceq_set_couplings(h, beta, beta_a, beta_2, beta_4), 'set_couplings')
   call lceq_check(lceq_set_gauge_fixing(h, lambda, alpha, limit_or, 10000000), 'set_gauge_fixing')
   call lceq_check(lceq_set_imode(h, imode), 'set_imode')
-  call lceq_check(lceq_rng_philox(h, seed, 0_c_int64_t), 'rng_philox')
+  if (ranlux_seed) then
+    ! f/measinit.f:214-217 on rank 0
+    call lceq_check(lceq_rng_measinit(h, 0, iseed), 'rng_measinit')
+    seed = lceq_measinit_key(iseed)
+  else
+    call lceq_check(lceq_rng_philox(h, seed, 0_c_int64_t), 'rng_philox')
+  end if
 
   ! ---- start configuration (f/confget.f:51-59)
   select case (init)
@@ -153,6 +170,10 @@ program lceq_main
   write(corra2_file, 1000) 'corra2', nl(3), betaint, xint, yint, confstart
   call header(u_act, action_file)
   call header(u_job, joblog_file)
+  open(u_job, file=joblog_file, status='old', position='append')
+  if (ranlux_seed) write(u_job, '(a,16(1x,i7))') '# iseed', iseed
+  write(u_job, '(a,1x,i20)') '# philox key', seed
+  close(u_job)
   call header(u_cor, corr_file)
   call header(u_afi, afield_file)
   call header(u_chk, checkp_file)

@@ -135,6 +135,14 @@ class Lattice:
     def rng_philox(self, seed, pass_counter=0):
         check(self._L.lceq_rng_philox(self.h, seed, pass_counter))
 
+    def rng_measinit(self, rank=0):
+        """Seed the device streams as f/measinit.f:214-217 seeds the
+        reference's random_number for MPI rank ``rank``; returns the 16
+        seeds (the Philox key is ranlux.measinit_key of them)."""
+        v = (C.c_int * 16)()
+        check(self._L.lceq_rng_measinit(self.h, int(rank), v))
+        return np.array(v[:], np.int32)
+
     def rng_feed(self, uniforms):
         u = np.ascontiguousarray(uniforms, dtype=np.float64)
         check(self._L.lceq_rng_feed(self.h, _dp(u), len(u)))

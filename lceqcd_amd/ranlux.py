@@ -71,3 +71,12 @@ def measinit_seeds(rank=0):
     v = (C.c_int * 16)()
     check(load().lceq_measinit_seeds(int(rank), v))
     return np.array(v[:], np.int32)
+
+
+def measinit_key(iseed):
+    """The Philox key the library derives from 16 measinit seeds
+    (lceq_measinit_key): iseed[0] + 2^32 iseed[1]."""
+    v = np.ascontiguousarray(np.asarray(iseed, dtype=np.int32))
+    if v.shape != (16,):
+        raise ValueError("measinit gives 16 seeds")
+    return int(load().lceq_measinit_key(v.ctypes.data_as(C.POINTER(C.c_int))))

@@ -34,7 +34,9 @@ def test_every_declared_symbol_is_exported(lib):
 
 def test_python_binding_covers_the_header():
     from lceqcd_amd import _lib
-    names = set(header_functions()) - {"lceq_last_error"}
+    # status-returning entry points; the two value-returning ones are bound
+    # apart with their own result types
+    names = set(header_functions()) - {"lceq_last_error", "lceq_measinit_key"}
     assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
 
 
@@ -44,6 +46,24 @@ def test_shape_validation_before_device_use(lib):
         rc = lib.lceq_create(*shape, C.byref(h))
         assert rc == -1 and not h.value
         assert b"even" in lib.lceq_last_error()
+
+
+def test_byte_offset_range_checked_before_device_use(lib):
+    """Device loads address a site of a class array by an unsigned 32-bit byte
+    offset s * 16, so more than 2^28 sites per parity must be refused before
+    any allocation (816^3 passes the 32-bit link-count check: 3 V < 2^31)."""
+    h = C.c_void_p()
+    assert 816 ** 3 * 3 < 2 ** 31 and 816 ** 3 // 2 > 2 ** 28
+    rc = lib.lceq_create(816, 816, 816, C.byref(h))
+    assert rc == -1 and not h.value
+    assert b"2^28" in lib.lceq_last_error()
+    dims = (C.c_int * 6)(16, 16, 16, 816, 816, 816)
+    b = C.c_void_p()
+    rc = lib.lceq_batch_create(2, dims, C.byref(b))
+    assert rc == -1 and not b.value
+    assert b"replica 1" in lib.lceq_last_error() and b"2^28" in lib.lceq_last_error()
+    # the largest accepted parity: 2^28 sites
+    assert 1024 * 1024 * 512 // 2 == 2 ** 28
 
 
 def test_null_handle_is_rejected(lib):

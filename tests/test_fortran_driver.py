@@ -156,3 +156,66 @@ def test_driver_polyakov_and_scalarcorr_outputs(tmp_path, oracle_lib, corrflag):
         if corrflag == 2:
             m, c = O.scalarcorr()
             np.testing.assert_allclose(ca2[it][1], [m] + list(c), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_driver_seeds_the_device_streams_from_ranlux(tmp_path, oracle_lib):
+    """Without a numeric seed the driver seeds its device streams as
+    f/measinit.f:214-217 seeds the reference's random_number on rank 0
+    (lceq_rng_measinit): the joblog lists the 16 seeds, which equal the ones
+    the reference's own f/luxury.f built here gives (tests/golden/
+    ranlux_ref.npz), and the run equals the CPU restatement's run of the
+    same schedule with the Philox key iseed(1) + 2^32 iseed(2)."""
+    exe = os.path.join(FDIR, "lceq_main")
+    (tmp_path / "inputfile").write_text(INPUTFILE.format(init=0, nsweep="00002"))
+    print("running lceq_main 8^3 nsweep=2 ntherm=0 with RANLUX seeding", flush=True)
+    r = subprocess.run([exe, "inputfile", "ranlux", "0"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    job = open(glob.glob(str(tmp_path / "joblog_s08_*"))[0]).read().splitlines()
+    seeds = [int(v) for v in next(l for l in job if l.startswith("# iseed")).split()[2:]]
+    key = int(next(l for l in job if l.startswith("# philox key")).split()[-1])
+    from helpers import golden
+    z = golden("ranlux_ref")
+    want = z["measinit_seeds"][list(z["measinit_ranks"]).index(0)]
+    assert seeds == [int(v) for v in want]
+    assert key == seeds[0] + (seeds[1] << 32)
+    lines = open(glob.glob(str(tmp_path / "action_s08_*"))[0]).read().splitlines()
+    rec = np.array([[float(x) for x in l.split()[1:]] for l in lines[3:]])
+    from lceqcd_amd import couplings_from_xy
+    O = oracle_lib.OracleLattice(8, 8, 8)
+    O.set_couplings(9.0, *couplings_from_xy(9.0, 0.1, 0.5))
+    O.hotstart(key, 0.5)
+    O.philox(key, 0)
+    exp = []
+    for it in range(2):
+        O.ahiggs2(1)
+        O.ahiggs3(1)
+        O.agauge(1)
+        O.renorm()
+        ob = O.observables()
+        exp.append([ob["plaq0"], ob["a0kin"], ob["a02"], ob["a04"]])
+        O.set_gauge_fixing(1.7, 1.0, 1e-10)
+        O.landau_gauge()
+        O.correlator()
+    np.testing.assert_allclose(rec, np.array(exp), rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_lattice_rng_measinit_matches_explicit_key():
+    """lceq_rng_measinit(rank) returns the rank's seeds and keys the streams
+    with measinit_key(seeds): the same chain as lceq_rng_philox(key, 0)."""
+    import lceqcd_amd
+    out = []
+    for how in ("measinit", "explicit"):
+        L = lceqcd_amd.Lattice(8, 8, 8)
+        L.set_couplings(9.0, 0.175, -0.45, 0.025)
+        if how == "measinit":
+            s = L.rng_measinit(5)
+            assert (s == lceqcd_amd.measinit_seeds(5)).all()
+        else:
+            L.rng_philox(lceqcd_amd.measinit_key(lceqcd_amd.measinit_seeds(5)), 0)
+        L.unitconf()
+        L.sweep(0, 3, n_or=4, delta=0.4, measflag=1)
+        out.append((L.get_links(), L.get_higgs()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

@@ -1,0 +1,108 @@
+"""The lattice shapes of the production and gauge-fixing configurations
+against the CPU restatement (oracle/lceq_oracle.c):
+
+- the elongated 32x32x128 lattice of the production runs, hot start: two
+  production sweeps (1 KP + 4 OR + ahiggs1, f/agauge.f, link
+  overrelaxation, f/ahiggs1.f) follow the restatement with no accept/reject
+  flip (1e-12 after one sweep, 1e-11 after two: the chain's own 1-ulp
+  sensitivity, tests/test_oracle_chaos.py), and further sweeps stay finite,
+  unitary and bitwise reproducible;
+- 32^3 Landau gauge fixing (f/landau_gauge.f, f/or_gauge_fixing1.f) with
+  lambda = 1.7, alpha = 1 to da < 1e-12 from a thermalised configuration:
+  the same iteration count as the restatement, da and the fixed fields
+  within rounding, and the z-correlators of f/correlator.f and
+  f/scalarcorr.f of the fixed fields after it."""
+import numpy as np
+import pytest
+
+from helpers import assert_close
+
+pytestmark = pytest.mark.gpu
+
+
+def _flips(got, exp):
+    d = np.abs(np.asarray(got) - np.asarray(exp)).max(axis=-1)
+    return int((d > 1e-9).sum()), float(d.max())
+
+
+def test_elongated_32x32x128_production_vs_oracle(oracle_lib):
+    import lceqcd_amd
+    from lceqcd_amd.workloads import coupling
+    shape = (32, 32, 128)
+    coup = coupling(0.33, -0.9)
+    print("32x32x128 production sweeps vs oracle", flush=True)
+    G = lceqcd_amd.Lattice(*shape)
+    O = oracle_lib.OracleLattice(*shape)
+    for L in (G, O):
+        L.set_couplings(*coup)
+        L.hotstart(12345, 0.5)
+        L.rng_philox(12345, 0)
+    O.set_renorm_on_write(1)
+    assert_close(G.get_links(), O.get_links(), 1e-14, "hot start links")
+    dG = dO = 0.4
+    for sweep, tol in ((1, 1e-12), (2, 1e-11)):
+        dG = G.sweep(0, 1, n_or=4, delta=dG, measflag=1)
+        O.agauge(0)
+        for k in range(4):
+            O.link_or(1 if k == 3 else 0)
+        dO, _ = O.ahiggs1(dO, 1)
+        assert dG == dO
+        fu, du = _flips(G.get_links(), O.get_links())
+        fa, da = _flips(G.get_higgs(), O.get_higgs())
+        print("sweep %d: links %.2e higgs %.2e flips %d %d" % (sweep, du, da, fu, fa), flush=True)
+        assert fu == 0 and fa == 0
+        assert_close(G.get_links(), O.get_links(), tol, "links")
+        assert_close(G.get_higgs(), O.get_higgs(), tol, "higgs")
+        h = G.history(1)[0]
+        ob = O.observables()
+        for i, key in ((0, "plaq0"), (2, "a02"), (3, "a04"), (1, "a0kin")):
+            assert_close(h[i], ob[key], tol, key)
+    # full size: 20 more sweeps twice from the same state, bit for bit
+    u0, a0 = G.get_links(), G.get_higgs()
+    runs = []
+    for rep in range(2):
+        X = lceqcd_amd.Lattice(*shape)
+        X.set_couplings(*coup)
+        X.put_links(u0)
+        X.put_higgs(a0)
+        X.rng_philox(777, 64)
+        X.sweep(0, 20, n_or=4, delta=dG, measflag=1)
+        runs.append((X.get_links(), X.get_higgs(), X.history(20)))
+        X.close()
+    for x, y in zip(runs[0], runs[1]):
+        assert np.array_equal(x, y)
+    u, a, h = runs[0]
+    assert np.isfinite(u).all() and np.isfinite(a).all() and np.isfinite(h).all()
+    assert np.abs(np.linalg.norm(u, axis=-1) - 1.0).max() < 1e-14
+
+
+def test_landau_gauge_32cubed_to_1e_12(oracle_lib):
+    import lceqcd_amd
+    from lceqcd_amd.workloads import C2, coupling
+    shape = C2["shape"]
+    print("32^3 landau gauge to 1e-12 vs oracle", flush=True)
+    G = lceqcd_amd.Lattice(*shape)
+    G.set_couplings(*coupling(C2["beta_a"], C2["beta_2"]))
+    G.hotstart(C2["seed"], 0.5)
+    G.rng_philox(C2["seed"], 0)
+    G.sweep(0, 50, n_or=4, delta=0.4, measflag=0)
+    O = oracle_lib.OracleLattice(*shape)
+    O.put_links(G.get_links())
+    O.put_higgs(G.get_higgs())
+    for L in (G, O):
+        L.set_gauge_fixing(C2["lam"], C2["alpha"], C2["limit_or"], 100000)
+    daG, itG = G.landau_gauge()
+    daO, itO = O.landau_gauge()
+    print("iterations GPU %d oracle %d, da %.4e / %.4e" % (itG, itO, daG, daO), flush=True)
+    assert itG == itO
+    assert daG <= C2["limit_or"] and daO <= C2["limit_or"]
+    assert_close(daG, daO, 1e-6, "final da")
+    assert_close(G.get_links_fixed(), O.get_links_fixed(), 1e-11, "u_f")
+    assert_close(G.get_higgs_fixed(), O.get_higgs_fixed(), 1e-11, "a_f")
+    # the correlators of the fixed fields (f/correlator.f) and of the
+    # unfixed Higgs field (f/scalarcorr.f)
+    for x, y, what in zip(G.correlator(), O.correlator(), ("C_a", "C_u", "C_3")):
+        assert_close(x, y, 1e-10, what)
+    (mg, cg), (mo, co) = G.scalarcorr(), O.scalarcorr()
+    assert_close(mg, mo, 1e-12, "scalar slice mean")
+    assert_close(cg, co, 1e-12, "scalar correlator")

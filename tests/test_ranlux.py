@@ -188,3 +188,18 @@ def test_default_initialisation_matches_rluxgo_default(lux):
     b.rluxgo(3, 314159265)
     assert (bits(a.ranlux(500)) == bits(b.ranlux(500))).all()
     assert a.rluxat() == (3, 314159265, 500 + 199 * (500 // 24), 0)
+
+
+def test_measinit_key_of_the_reference_seeds():
+    """The device streams' key from a rank's measinit seeds: iseed(1) +
+    2^32 iseed(2) (both below 10^6, so distinct seed pairs give distinct keys)."""
+    from helpers import golden
+    from lceqcd_amd import measinit_key
+    z = golden("ranlux_ref")
+    keys = set()
+    for rank, s in zip(z["measinit_ranks"], z["measinit_seeds"]):
+        k = measinit_key(s)
+        assert k == int(s[0]) + (int(s[1]) << 32)
+        assert 0 <= s[0] < 10 ** 6 and 0 <= s[1] < 10 ** 6
+        keys.add(k)
+    assert len(keys) == len(z["measinit_ranks"])
