@@ -14,7 +14,12 @@ Errors: Madras-Sokal integrated autocorrelation times for the means; a
 delete-one-block jackknife (30 blocks) for the connected correlator, with the
 ensemble mean subtracted, and for the cosh effective mass between
 separations 1 and 2.  The seeds are fixed, so the outcome is deterministic;
-every z-score is printed.  The bar is |z| < 3 for each quantity."""
+every z-score is printed.  The bar is |z| < 2 for each quantity, the
+north-star criterion (measured: plaquette -1.76, rho +1.16, Polyakov loop
++0.24, C(0..3) -1.69, +0.63, +0.45, +1.07, screening mass -0.71).  From two
+independent hot starts the plaquettes would differ by about 9 sigma on their
+own (tests/test_oracle_chaos.py), which is why both chains start from one
+thermalised configuration."""
 import numpy as np
 import pytest
 
@@ -102,7 +107,7 @@ def test_independent_chains_agree(oracle_lib):
               % (mg, emg, mo, emo, zs["screening mass"]), flush=True)
     else:  # C(0) + C(2) <= 2 C(1): no cosh mass at this statistics
         print("screening mass undefined: gpu %s cpu %s" % (mg, mo), flush=True)
-    bad = {k: v for k, v in zs.items() if not abs(v) < 3.0}
+    bad = {k: v for k, v in zs.items() if not abs(v) < 2.0}
     assert not bad, bad
 
 
@@ -113,7 +118,8 @@ def test_learned_proposal_chain_agrees_with_plain():
     independent Philox streams, give the same <plaquette>, <Tr A^2> and
     <a.F> within errors (Madras-Sokal tau_int).  Exactness for any weights
     is what this checks at beta_A != 0; the beta_A = 0 limit is checked
-    against quadrature in test_gpu_mlp.py."""
+    against quadrature in test_gpu_mlp.py.  Bar |z| < 2 (measured +0.52,
+    -0.94, -1.19)."""
     import lceqcd_amd
     from lceqcd_amd.analysis import mean_error
     shape = (16, 16, 16)
@@ -152,7 +158,7 @@ def test_learned_proposal_chain_agrees_with_plain():
         z = (m1 - m0) / np.hypot(e0, e1)
         print("%-10s plain %.6g +- %.2g  learned %.6g +- %.2g  z = %+.2f" % (nm, m0, e0, m1, e1, z),
               flush=True)
-        if not abs(z) < 3.0:
+        if not abs(z) < 2.0:
             bad[nm] = z
     acc = rows[True][:, 8].mean()
     print("learned acceptance %.3f plain %.3f" % (acc, rows[False][:, 8].mean()), flush=True)
