@@ -22,7 +22,14 @@ block, XCD-aware block order) from 10 sweeps per shape and restores the chain
 Prints one JSON line.  value = site-updates/s (V sites per sweep).  The
 roofline object also carries copy_ceiling: the rate of the library's plain
 device copy (lceq_copy_bandwidth) moving the OR launch's own bytes, and of a
-1 GiB copy, timed in the same run.
+1 GiB copy, timed in the same run; hbm_resident: the same OR launch timed on
+a 160^3 lattice (490 MB, more than the 256 MB Infinity Cache holds, which the
+64^3 field set does not exceed); and the source of its traffic figure.  The
+learned object compares the learned Higgs proposal in batched form with the
+plain sweep on the same replicas (4 x 48^3 at beta_A = 0.34 in the broken
+phase and 4 x 48^3 near the transition, 20000 sweeps each, one network):
+tau_int of Tr A^2, acceptance, independent samples/s and their ratio, also
+with the collection and training time charged (--no-learned leaves it out).
 
     python bench.py --workload replicas96 [--steps K] [--hist-out FILE]
 
@@ -54,6 +61,27 @@ last configuration, timed per iteration, and the three z-correlator
 measurements; each dominant launch is timed with HIP events.  cpu_baseline:
 the reference's own routines (oracle/_ref/refharness64_64, built from its
 sources with 8-byte reals) timed on the same configuration, when built.
+
+    python bench.py --workload scan [--scan-lattice 24,24,24] [--therm T] [--steps K]
+
+<Tr A^2> against beta_2 at beta_A = 0.28, 0.34, 0.40 (96 replicas, one batch):
+locates the symmetric/broken transition.
+
+    python bench.py --workload mlpbatch
+
+only the batched learned-vs-plain comparison of the default line.
+
+    python bench.py --workload plain | gaugefix32 | production
+
+the benchmark configurations as runs (lceqcd_amd.runs): 16^3 cold start with
+per-sweep plaquette, Tr A^2 and Polyakov loop; 10 saved 32^3 configurations
+Landau-fixed to 1e-12 with the correlators; 64^3 and 32^2x128 at beta_A =
+0.30, 0.33, 0.36, 5000 sweeps with gauge fixing and correlators every 50.
+
+    python bench.py --workload kernels [--lattice L,L,L] [--steps K]
+
+each routine K times on one lattice: the command the per-kernel rocprofv3
+counter passes profile (a few hundred dispatches).
 
     python bench.py --calibrate-copy
 
@@ -89,7 +117,7 @@ OR_TRAFFIC_64 = {512: 33.66e6 + 4.03e6, 256: 33.68e6 + 4.04e6, 128: 33.68e6 + 4.
 # couplings (trace convention) of the batched learned-vs-plain comparison:
 # the broken phase of the learned-proposal configuration, and one near the
 # symmetric/broken transition
-LEARNED_COUPLINGS = "0.34,-0.9;0.34,0.3"
+LEARNED_COUPLINGS = "0.34,-0.9;0.34,0.2"
 
 
 def couplings(beta_a_trace=0.33):
@@ -163,20 +191,15 @@ def cpu_baseline_slabs(shape, seconds, n_or, procs):
                          max(k for k, _ in res), procs)}
 
 
-REPLICA_SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 32), (40, 40, 40), (48, 48, 48),
-                  (24, 24, 96)]
-
 
 def replicas96(args):
     import numpy as np
     import lceqcd_amd
-    from lceqcd_amd import couplings_from_trace_convention
+    from lceqcd_amd.workloads import C4_BETA_A, C4_BETA_2, C4_STRADDLE_BETA_2, c4_replicas
     nrep = 96
-    shapes = [REPLICA_SHAPES[r % len(REPLICA_SHAPES)] for r in range(nrep)]
-    ba = np.linspace(0.28, 0.40, 12)
-    b2 = np.linspace(-1.2, -0.6, 8)
-    coup = [couplings_from_trace_convention(9.0, ba[r % 12], b2[r // 12], 0.1) for r in range(nrep)]
-    seeds = [12345 + r for r in range(nrep)]
+    ba = C4_BETA_A
+    b2 = C4_STRADDLE_BETA_2 if args.grid == "straddle" else C4_BETA_2
+    shapes, coup, seeds, _ = c4_replicas(nrep, ba, b2)
     B = lceqcd_amd.Batch(shapes)
     B.set_couplings(coup)
     B.hotstart(seeds, 0.5)
@@ -214,6 +237,8 @@ def replicas96(args):
             "workload": "96 replicas (16^3..48^3, 24^2x96) on a beta_A x beta_2 grid, 1 KP + %d OR "
                         "+ ahiggs1 per sweep, hipGraph of 8 sweeps, autotuned launch shape"
                         % args.n_or,
+            "grid": {"beta_a": [float(x) for x in ba], "beta_2": [float(x) for x in b2],
+                     "kind": args.grid},
             "total_sites": V,
             "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
             "sweep_algorithmic_frac_of_hbm": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9
@@ -672,6 +697,46 @@ def hbm_resident_or(shape, n_or, tb_x, sweeps=10):
             "achieved": gbps, "frac": gbps / HBM_PEAK_GBPS}
 
 
+def kernels(args):
+    """Each routine of the update and measurement path a few times on one
+    lattice (default 64^3), for per-kernel profiles: rocprofv3 --pmc
+    serialises every dispatch, so a profiled command must stay short.  The
+    JSON line gives the host time per call (launch and synchronisation
+    included); the kernel durations come from the profiler's trace."""
+    import lceqcd_amd
+    from lceqcd_amd import couplings_from_xy
+    shape = tuple(int(x) for x in args.lattice.split(","))
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(9.0, *couplings_from_xy(9.0, 0.1, 0.5))
+    L.set_imode(0)
+    L.hotstart(314159, 0.5)
+    L.rng_philox(314159, 0)
+    if args.shape:
+        L.set_shape(*(int(v) for v in args.shape.split(",")))
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, 2, measflag=1)
+    L.history(2)
+    L.set_gauge_fixing(1.7, 1.0, 1e-10)
+    L.put_links_fixed(L.get_links())
+    L.put_higgs_fixed(L.get_higgs())
+    calls = [("ahiggs2", lambda: L.ahiggs2(1)), ("ahiggs3", lambda: L.ahiggs3(1)),
+             ("ahiggs1", lambda: L.ahiggs1(0.4, 1)), ("agauge", lambda: L.agauge(1)),
+             ("link_or", lambda: L.link_or(1)), ("renorm", L.renorm),
+             ("or_gauge_fixing", L.or_gauge_fixing), ("u_f_renorm", L.u_f_renorm),
+             ("correlator", L.correlator), ("scalarcorr", L.scalarcorr),
+             ("polyakov", L.polyakov)]
+    out = {}
+    for name, fn in calls:
+        fn()
+        L.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        L.sync()
+        out[name] = 1e3 * (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"workload": "kernels", "lattice": list(shape), "calls": args.steps,
+                      "host_ms_per_call": out}), flush=True)
+
+
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -738,7 +803,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64", "scan",
-                             "mlpbatch", "plain", "gaugefix32", "production"])
+                             "mlpbatch", "plain", "gaugefix32", "production", "kernels"])
     ap.add_argument("--sweeps", type=int, default=None,
                     help="sweeps per chain (learned workloads; default 60000 at 48^3, "
                          "20000 at 96^3: a few % statistical error on tau_int)")
@@ -748,6 +813,9 @@ def main():
                     help="thermalisation sweeps before training (learned workloads)")
     ap.add_argument("--hist-out", default=None)
     ap.add_argument("--scan-lattice", default="24,24,24")
+    ap.add_argument("--grid", default="literal", choices=["literal", "straddle"],
+                    help="replicas96: the configuration's beta_2 grid (broken phase "
+                         "throughout) or one that straddles the transition")
     ap.add_argument("--no-learned", action="store_true",
                     help="leave the batched learned-vs-plain object out of the default line")
     ap.add_argument("--learned-lattice", type=int, default=48)
@@ -774,6 +842,8 @@ def main():
         return refsched64(args)
     if args.workload == "scan":
         return scan(args)
+    if args.workload == "kernels":
+        return kernels(args)
     if args.workload == "mlpbatch":
         print(json.dumps({"workload": "mlpbatch", "learned": learned_batch(args, log=_log)}),
               flush=True)

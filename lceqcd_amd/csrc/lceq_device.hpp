@@ -216,11 +216,27 @@ __device__ __forceinline__ su2 ld_link(const double* __restrict__ U, const Geom&
   double2 lo = at(q, s), hi = at(q + g.npad, s);
   return su2{c_mk(lo.x, lo.y), c_mk(hi.x, hi.y)};
 }
+// Stores write through the L2 (device-scope cache policy sc1): every launch
+// ends with the L2s of the eight XCDs written back, so that the next,
+// dependent launch sees the updated fields on every XCD; stored through as
+// they are made, the updated links are already out when the last wave ends.
+// Raw buffer stores take the policy bits; the range check (offset + size <=
+// 2^32 - 1) holds because a parity has fewer than 2^28 sites.
+typedef int lceq_v4i __attribute__((ext_vector_type(4)));
+typedef int lceq_v2i __attribute__((ext_vector_type(2)));
+constexpr int CPOL_WRITE_THROUGH = 16;  // sc1
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, -1, 0x00020000);
+}
 __device__ __forceinline__ void st_link(double* __restrict__ U, const Geom& g, int mu, int p,
                                         unsigned s, su2 u) {
   double2* q = reinterpret_cast<double2*>(U + (size_t)((mu * 2 + p) * 4) * (size_t)g.npad);
-  at(q, s) = make_double2(u.a.re, u.a.im);
-  at(q + g.npad, s) = make_double2(u.b.re, u.b.im);
+  __builtin_amdgcn_raw_buffer_store_b128(
+      __builtin_bit_cast(lceq_v4i, make_double2(u.a.re, u.a.im)), raw_rsrc(q), s * 16u, 0,
+      CPOL_WRITE_THROUGH);
+  __builtin_amdgcn_raw_buffer_store_b128(
+      __builtin_bit_cast(lceq_v4i, make_double2(u.b.re, u.b.im)), raw_rsrc(q + g.npad), s * 16u,
+      0, CPOL_WRITE_THROUGH);
 }
 __device__ __forceinline__ vec3 ld_higgs(const double* __restrict__ A, const Geom& g, int p,
                                          unsigned s) {
@@ -231,8 +247,11 @@ __device__ __forceinline__ vec3 ld_higgs(const double* __restrict__ A, const Geo
 __device__ __forceinline__ void st_higgs(double* __restrict__ A, const Geom& g, int p, unsigned s,
                                          const vec3& a) {
   double* q = A + (size_t)(p * 3) * (size_t)g.npad;
-  at(reinterpret_cast<double2*>(q), s) = make_double2(a.x[0], a.x[1]);
-  at(q + 2 * (size_t)g.npad, s) = a.x[2];
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(lceq_v4i, make_double2(a.x[0], a.x[1])),
+                                         raw_rsrc(q), s * 16u, 0, CPOL_WRITE_THROUGH);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lceq_v2i, a.x[2]),
+                                        raw_rsrc(q + 2 * (size_t)g.npad), s * 8u, 0,
+                                        CPOL_WRITE_THROUGH);
 }
 
 // Offset in doubles of component c (0..ncomp-1) of site s in class cls of a

@@ -74,15 +74,16 @@ int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3
 // Besides even extents >= 2: 3 V links fit the 32-bit link counters, the
 // index divisions need ly, lz < 2^22, and the device loads address a site
 // of a class array as an unsigned 32-bit byte offset s * 16 (16-byte link
-// halves), so a parity holds at most 2^28 sites (N <= 2^28 keeps
-// (npad - 1) * 16 below 2^32).  Checked before any device call.
+// halves) and the stores' raw buffer range is 2^32 - 1 bytes, so a parity
+// holds fewer than 2^28 sites (s * 16 + 16 <= 2^32 - 16).  Checked before
+// any device call.
 inline const char* geom_reject(int lx, int ly, int lz) {
   if (lx < 2 || ly < 2 || lz < 2 || (lx & 1) || (ly & 1) || (lz & 1))
     return "extents must be even and >= 2";
   if ((int64_t)lx * ly * lz * 3 > INT32_MAX) return "lattice too large for 32-bit link counters";
   if (ly >= (1 << 22) || lz >= (1 << 22)) return "ly and lz must be below 2^22";
-  if ((int64_t)lx * ly * lz / 2 > ((int64_t)1 << 28))
-    return "more than 2^28 sites per parity (32-bit byte offsets of 16-byte link halves)";
+  if ((int64_t)lx * ly * lz / 2 >= ((int64_t)1 << 28))
+    return "2^28 or more sites per parity (32-bit byte offsets of 16-byte link halves)";
   return nullptr;
 }
 

@@ -39,9 +39,17 @@ C4_SHAPES = [(16, 16, 16), (24, 24, 24), (32, 32, 32), (40, 40, 40), (48, 48, 48
              (24, 24, 96)]
 C4_BETA_A = np.linspace(0.28, 0.40, 12)
 C4_BETA_2 = np.linspace(-1.2, -0.6, 8)
+# The symmetric/broken transition (24^3 scan, bench.py --workload scan:
+# steepest fall of <Tr A^2> and peak of its susceptibility) lies at beta_2
+# of about -0.1, 0.2 and 0.5 for beta_A = 0.28, 0.34 and 0.40, so the grid
+# above is broken phase throughout; this beta_2 range straddles it at every
+# beta_A of the grid.
+C4_STRADDLE_BETA_2 = np.linspace(-0.6, 1.2, 8)
 
-# C5: learned proposal near the transition
-C5 = {"shapes": [(48, 48, 48), (96, 96, 96)], "beta_a": 0.34, "beta_2": -0.9}
+# C5: learned proposal at beta_A = 0.34: the configuration's beta_2 = -0.9
+# (broken phase, <Tr A^2> about 9) and beta_2 = 0.2, at the transition
+C5 = {"shapes": [(48, 48, 48), (96, 96, 96)], "beta_a": 0.34, "beta_2": -0.9,
+      "beta_2_transition": 0.2}
 
 
 def coupling(beta_a, beta_2, beta_4=BETA_4, beta_g=BETA_G):
@@ -54,8 +62,8 @@ def c4_replicas(nrep=96, beta_a=None, beta_2=None):
     shape C4_SHAPES[r % 6], beta_A = beta_a[r % len(beta_a)], beta_2 =
     beta_2[r // len(beta_a)] (trace convention), seed 12345 + r.  The default
     grid is the configuration's literal one (all of it in the broken phase at
-    beta_4 = 0.1, where <Tr A^2> is 6 to 10); pass another grid to cross
-    the transition."""
+    beta_4 = 0.1, where <Tr A^2> is 6 to 10); C4_STRADDLE_BETA_2 as beta_2
+    crosses the transition."""
     ba = C4_BETA_A if beta_a is None else np.asarray(beta_a, dtype=np.float64)
     b2 = C4_BETA_2 if beta_2 is None else np.asarray(beta_2, dtype=np.float64)
     if nrep > len(ba) * len(b2):

@@ -50,8 +50,9 @@ def test_shape_validation_before_device_use(lib):
 
 def test_byte_offset_range_checked_before_device_use(lib):
     """Device loads address a site of a class array by an unsigned 32-bit byte
-    offset s * 16, so more than 2^28 sites per parity must be refused before
-    any allocation (816^3 passes the 32-bit link-count check: 3 V < 2^31)."""
+    offset s * 16 (and stores through a raw buffer of 2^32 - 1 bytes), so
+    2^28 or more sites per parity must be refused before any allocation
+    (816^3 passes the 32-bit link-count check: 3 V < 2^31)."""
     h = C.c_void_p()
     assert 816 ** 3 * 3 < 2 ** 31 and 816 ** 3 // 2 > 2 ** 28
     rc = lib.lceq_create(816, 816, 816, C.byref(h))
@@ -62,8 +63,9 @@ def test_byte_offset_range_checked_before_device_use(lib):
     rc = lib.lceq_batch_create(2, dims, C.byref(b))
     assert rc == -1 and not b.value
     assert b"replica 1" in lib.lceq_last_error() and b"2^28" in lib.lceq_last_error()
-    # the largest accepted parity: 2^28 sites
-    assert 1024 * 1024 * 512 // 2 == 2 ** 28
+    # exactly 2^28 sites per parity (1024 x 1024 x 512) is refused as well
+    rc = lib.lceq_create(1024, 1024, 512, C.byref(h))
+    assert rc == -1 and not h.value and b"2^28" in lib.lceq_last_error()
 
 
 def test_null_handle_is_rejected(lib):

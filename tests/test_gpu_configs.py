@@ -9,9 +9,10 @@ against the CPU restatement (oracle/lceq_oracle.c):
   unitary and bitwise reproducible;
 - 32^3 Landau gauge fixing (f/landau_gauge.f, f/or_gauge_fixing1.f) with
   lambda = 1.7, alpha = 1 to da < 1e-12 from a thermalised configuration:
-  the same iteration count as the restatement, da and the fixed fields
-  within rounding, and the z-correlators of f/correlator.f and
-  f/scalarcorr.f of the fixed fields after it."""
+  the same iteration count as the restatement, the gauge-invariant |a_f| to
+  rounding, and the fixed fields and the z-correlators of f/correlator.f
+  within the restatement's own sensitivity to rounding; f/scalarcorr.f of
+  the unfixed Higgs field to rounding."""
 import numpy as np
 import pytest
 
@@ -76,7 +77,36 @@ def test_elongated_32x32x128_production_vs_oracle(oracle_lib):
     assert np.abs(np.linalg.norm(u, axis=-1) - 1.0).max() < 1e-14
 
 
+def _oracle_landau_with_noise(oracle_lib, shape, u, a, limit, seed):
+    """The restatement's landau_gauge loop (f/landau_gauge.f:54-59) with the
+    fixed links perturbed by +-1 ulp after every iteration: the sensitivity of
+    the gauge-fixing iteration to rounding."""
+    X = oracle_lib.OracleLattice(*shape)
+    X.set_gauge_fixing(1.7, 1.0, limit, 100000)
+    X.put_links_fixed(u)
+    X.put_higgs_fixed(a)
+    rng = np.random.default_rng(seed)
+    it, da = 0, 1e4
+    while da > limit and it < 100000:
+        da = X.or_gauge_fixing()
+        X.u_f_renorm()
+        it += 1
+        uf = X.get_links_fixed()
+        X.put_links_fixed(uf * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], size=uf.shape)))
+    return X, da, it
+
+
 def test_landau_gauge_32cubed_to_1e_12(oracle_lib):
+    """Gauge fixing to da < 1e-12 fixes the gauge only up to slowly decaying
+    long-wavelength modes: the iteration stops about sqrt(da) / lambda_min
+    from the exact Landau gauge, and any two rounding paths land at
+    different points inside that region.  The restatement perturbed by 1 ulp
+    after every iteration drifts by ~5e-8 in the links and ~3e-7 in a_f at
+    32^3 from its unperturbed run, while the gauge-invariant |a_f| per site
+    stays within 1e-12.  So the GPU must reach da < 1e-12 in the same number
+    of iterations, agree with the restatement in the gauge-invariant |a_f| to
+    1e-11, and stay within 10 times that 1-ulp envelope in the fixed fields and
+    the gauge-fixed correlators."""
     import lceqcd_amd
     from lceqcd_amd.workloads import C2, coupling
     shape = C2["shape"]
@@ -86,23 +116,36 @@ def test_landau_gauge_32cubed_to_1e_12(oracle_lib):
     G.hotstart(C2["seed"], 0.5)
     G.rng_philox(C2["seed"], 0)
     G.sweep(0, 50, n_or=4, delta=0.4, measflag=0)
+    u, a = G.get_links(), G.get_higgs()
     O = oracle_lib.OracleLattice(*shape)
-    O.put_links(G.get_links())
-    O.put_higgs(G.get_higgs())
+    O.put_links(u)
+    O.put_higgs(a)
     for L in (G, O):
         L.set_gauge_fixing(C2["lam"], C2["alpha"], C2["limit_or"], 100000)
     daG, itG = G.landau_gauge()
     daO, itO = O.landau_gauge()
-    print("iterations GPU %d oracle %d, da %.4e / %.4e" % (itG, itO, daG, daO), flush=True)
+    P, daP, itP = _oracle_landau_with_noise(oracle_lib, shape, u, a, C2["limit_or"], 1)
+    print("iterations GPU %d oracle %d noisy oracle %d, da %.6e / %.6e / %.6e"
+          % (itG, itO, itP, daG, daO, daP), flush=True)
     assert itG == itO
     assert daG <= C2["limit_or"] and daO <= C2["limit_or"]
-    assert_close(daG, daO, 1e-6, "final da")
-    assert_close(G.get_links_fixed(), O.get_links_fixed(), 1e-11, "u_f")
-    assert_close(G.get_higgs_fixed(), O.get_higgs_fixed(), 1e-11, "a_f")
-    # the correlators of the fixed fields (f/correlator.f) and of the
-    # unfixed Higgs field (f/scalarcorr.f)
-    for x, y, what in zip(G.correlator(), O.correlator(), ("C_a", "C_u", "C_3")):
-        assert_close(x, y, 1e-10, what)
+    uG, uO, uP = G.get_links_fixed(), O.get_links_fixed(), P.get_links_fixed()
+    aG, aO, aP = G.get_higgs_fixed(), O.get_higgs_fixed(), P.get_higgs_fixed()
+    env_u, env_a = np.abs(uP - uO).max(), np.abs(aP - aO).max()
+    dev_u, dev_a = np.abs(uG - uO).max(), np.abs(aG - aO).max()
+    inv = np.abs(np.linalg.norm(aG, axis=-1) - np.linalg.norm(aO, axis=-1)).max()
+    print("fixed fields: links %.2e (1-ulp envelope %.2e), a_f %.2e (%.2e), |a_f| %.2e"
+          % (dev_u, env_u, dev_a, env_a, inv), flush=True)
+    assert inv < 1e-11
+    assert dev_u <= 10.0 * env_u + 1e-13 and dev_a <= 10.0 * env_a + 1e-13
+    assert abs(daG - daO) <= 10.0 * abs(daP - daO) + 1e-6 * daO
+    # the correlators of the fixed fields (f/correlator.f), within 10 times the
+    # envelope, and of the unfixed Higgs field (f/scalarcorr.f) to rounding
+    for x, y, z, what in zip(G.correlator(), O.correlator(), P.correlator(),
+                             ("C_a", "C_u", "C_3")):
+        env = np.abs(z - y).max()
+        print("%s: GPU %.2e envelope %.2e" % (what, np.abs(x - y).max(), env), flush=True)
+        assert np.abs(x - y).max() <= 10.0 * env + 1e-12 * np.abs(y).max()
     (mg, cg), (mo, co) = G.scalarcorr(), O.scalarcorr()
     assert_close(mg, mo, 1e-12, "scalar slice mean")
     assert_close(cg, co, 1e-12, "scalar correlator")

@@ -171,6 +171,7 @@ def test_philox_production_sweeps_16(make, omake):
         d, _ = O.ahiggs1(d, 1)
         dp, _ = P.ahiggs1(dp, 1)
         dG = G.sweep(0, 1, n_or=4, delta=dG, measflag=1)
+        h = G.history(1)
         assert dG == d
         env = float(np.abs(O.get_links() - P.get_links()).max())
         dev_u, flips_u = _compare_sites(G.get_links(), O.get_links(), "links after %d" % it)
@@ -183,7 +184,6 @@ def test_philox_production_sweeps_16(make, omake):
         if it in bars:
             assert_close(G.get_links(), O.get_links(), bars[it], "links")
             assert_close(G.get_higgs(), O.get_higgs(), bars[it], "higgs")
-            h = G.history(it)
             assert_close(h[-1, 2], O.observables()["a02"], bars[it], "a02")
 
 
