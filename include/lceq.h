@@ -159,6 +159,17 @@ int lceq_renorm(lceq_lattice *L);
  * infinite link and Higgs values.  Returns LCEQ_ERR_FIELD when nonfinite > 0
  * or, for tol >= 0, when max_dev > tol.  Either output may be NULL. */
 int lceq_check_fields(lceq_lattice *L, double tol, double *max_dev, int64_t *nonfinite);
+/* lceq_check_fields without memory the library allocates.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-block partial sums and their one-block reduction live in a fixed 1 MiB
+ * device array per GPU, held until the process ends, instead of the scratch
+ * lceq_check_fields allocates on first use.  Lattices of at most 2^24 sites
+ * per parity (2^16 blocks of 256 sites) are accepted; larger ones return
+ * LCEQ_ERR_ARG before any device work.  Outputs and status equal
+ * lceq_check_fields'.  Calls that overlap on one GPU (from two threads, or on
+ * two lattices' streams) share the array and race. */
+int lceq_check_fields_static(lceq_lattice *L, double tol, double *max_dev, int64_t *nonfinite);
 /* u_f_renorm(), f/u_f_renorm.f:6 (called at f/landau_gauge.f:57) */
 int lceq_u_f_renorm(lceq_lattice *L);
 /* or_gauge_fixing(da1), f/or_gauge_fixing1.f:5 (called at f/landau_gauge.f:56) */

@@ -89,6 +89,7 @@ SIGNATURES = {
     "lceq_mlp_eval": [VP, C.POINTER(C.c_float), I, C.POINTER(C.c_float)],
     "lceq_mlp_set_sites_per_wave": [VP, I],
     "lceq_check_fields": [VP, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
+    "lceq_check_fields_static": [VP, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "lceq_higgs_mlp": [VP, VP, DP],
     "lceq_sweep_mlp": [VP, VP, I, I, I],
     "lceq_conf_write": [IP, DP, DP, C.c_char_p, C.c_char_p, I],

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = [os.path.join(HERE, "csrc", f) for f in ("lceq_kernels.hip", "lceq_api.hip", "lceq_batch.hip",
                                                    "lceq_ranlux.hip", "lceq_conf.hip",
-                                                   "lceq_mlp.hip")]
+                                                   "lceq_mlp.hip", "lceq_static.hip")]
 OUT = os.path.join(HERE, "liblceq.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
