@@ -575,8 +575,8 @@ void launch_gfix_finalize(lceq_lattice* L, int guarded, int iter) {
 // ========================================================== correlators
 
 // z-slice sums over both parities (f/correlator.f:53-79, f/scalarcorr.f:43-56,
-// f/polyakov.f:172-185), each slice split over `split` blocks (about 512
-// sites each, corr_split).  Quantities q (partial j at
+// f/polyakov.f:47-58), each slice split over `split` blocks (one site per
+// thread where a slice has at most 16 x 256 sites, corr_split).  Quantities q (partial j at
 // slices[(q*lz + z)*split + j]):
 //   0-2 a_f(1..3); 3+3*mu: Im alpha_mu, 4+3*mu: Im beta_mu, 5+3*mu: -Re beta_mu
 //   (the (Im a, -i b) projection of u_f); 12 rho(a); 13 cos(4|a|).
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(256) void k_slices(Geom g, int split,
 }
 
 // Periodic z-correlators on the slice means (f/correlator.f:139-164,
-// f/scalarcorr.f:96-104, f/polyakov.f:225-233).  corr layout:
+// f/scalarcorr.f:96-104, f/polyakov.f:97-107).  corr layout:
 // [0,lz) C_a, [lz,2lz) C_u, [2lz,3lz) C_3, [3lz,4lz) rho, [4lz,5lz) Polyakov,
 // then mean(rho), mean(P)^2, a0f(1..3).  The slice partials are summed in a
 // fixed order into slice means, held in LDS (dynamic, 14 lz doubles) when
@@ -720,8 +720,9 @@ constexpr size_t CORR_LDS_MAX = 64 * 1024;
 int corr_split(const Geom& g) {
   if (14 * (size_t)g.lz * sizeof(double) > CORR_LDS_MAX) return 1;
   int nz = 2 * g.h2;
-  // 512 sites per block, at most CORR_MAX_SPLIT blocks per slice
-  return std::max(1, std::min(CORR_MAX_SPLIT, (nz + 2 * TB - 1) / (2 * TB)));
+  // one site per thread (every load of a block in flight at once), at most
+  // CORR_MAX_SPLIT blocks per slice; larger slices loop
+  return std::max(1, std::min(CORR_MAX_SPLIT, (nz + TB - 1) / TB));
 }
 
 void launch_slices(lceq_lattice* L) {
