@@ -575,8 +575,8 @@ void launch_gfix_finalize(lceq_lattice* L, int guarded, int iter) {
 // ========================================================== correlators
 
 // z-slice sums over both parities (f/correlator.f:53-79, f/scalarcorr.f:43-56,
-// f/polyakov.f:47-58), each slice split over `split` blocks (one site per
-// thread where a slice has at most 16 x 256 sites, corr_split).  Quantities q (partial j at
+// f/polyakov.f:47-58), each slice split over `split` blocks (about 512
+// sites each, corr_split).  Quantities q (partial j at
 // slices[(q*lz + z)*split + j]):
 //   0-2 a_f(1..3); 3+3*mu: Im alpha_mu, 4+3*mu: Im beta_mu, 5+3*mu: -Re beta_mu
 //   (the (Im a, -i b) projection of u_f); 12 rho(a); 13 cos(4|a|).
@@ -720,9 +720,11 @@ constexpr size_t CORR_LDS_MAX = 64 * 1024;
 int corr_split(const Geom& g) {
   if (14 * (size_t)g.lz * sizeof(double) > CORR_LDS_MAX) return 1;
   int nz = 2 * g.h2;
-  // one site per thread (every load of a block in flight at once), at most
-  // CORR_MAX_SPLIT blocks per slice; larger slices loop
-  return std::max(1, std::min(CORR_MAX_SPLIT, (nz + TB - 1) / TB));
+  // 512 sites per block, at most CORR_MAX_SPLIT blocks per slice.  One site
+  // per thread (twice the blocks) was measured slower: 64^3 slice sums
+  // 11.3 -> 13.4 us (twice the block reductions) and the correlator block
+  // 12.5 -> 17.3 us (twice the partials per slice)
+  return std::max(1, std::min(CORR_MAX_SPLIT, (nz + 2 * TB - 1) / (2 * TB)));
 }
 
 void launch_slices(lceq_lattice* L) {
