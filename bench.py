@@ -673,10 +673,11 @@ def learned_batch(args, log=None):
     return res
 
 
-def hbm_resident_or(shape, n_or, tb_x, sweeps=10):
+def hbm_resident_or(shape, n_or, tb_x, sweeps=10, order=0):
     """The overrelaxation launch on a lattice too large for the 256 MB
     Infinity Cache (160^3: 490 MB of fields, 557 MB moved per launch), timed
-    with HIP events as the 64^3 launch is, at the same launch shape."""
+    with HIP events as the 64^3 launch is, at the same launch shape and site
+    order `order` (-1: the one the site-order tuner picks)."""
     import lceqcd_amd
     V = shape[0] * shape[1] * shape[2]
     L = lceqcd_amd.Lattice(*shape)
@@ -685,16 +686,23 @@ def hbm_resident_or(shape, n_or, tb_x, sweeps=10):
     L.rng_philox(12345, 0)
     L.set_shape(*tb_x)
     d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 2, n_or=n_or, delta=0.4, measflag=0)
+    if order < 0:
+        orders = L.autotune_site_order(n_or=n_or, trials=3)
+    else:
+        L.set_site_order(order)
+        orders = {}
     L.profile_begin(lceqcd_amd.KC_LINK_OR, 6 * n_or * sweeps)
     L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, sweeps, n_or=n_or, delta=d, measflag=1)
     ms, n = L.profile_end()
     L.history(sweeps)
+    so = L.get_site_order()
     L.close()
     us = ms * 1e3 / max(n, 1)
     b = LINK_PASS_BYTES_PER_SITE * V
     gbps = b / (us * 1e-6) / 1e9
     return {"lattice": list(shape), "bytes_per_launch": b, "avg_launch_us": us, "launches": n,
-            "achieved": gbps, "frac": gbps / HBM_PEAK_GBPS}
+            "achieved": gbps, "frac": gbps / HBM_PEAK_GBPS,
+            "site_order": so, "site_order_ms_per_sweep": orders}
 
 
 def kernels(args):
@@ -713,6 +721,8 @@ def kernels(args):
     L.rng_philox(314159, 0)
     if args.shape:
         L.set_shape(*(int(v) for v in args.shape.split(",")))
+    if args.order > 0:
+        L.set_site_order(args.order)
     L.sweep(lceqcd_amd.SWEEP_REFERENCE, 2, measflag=1)
     L.history(2)
     L.set_gauge_fixing(1.7, 1.0, 1e-10)
@@ -813,6 +823,11 @@ def main():
                     help="thermalisation sweeps before training (learned workloads)")
     ap.add_argument("--hist-out", default=None)
     ap.add_argument("--scan-lattice", default="24,24,24")
+    ap.add_argument("--order", type=int, default=0,
+                    help="site order (strip rows, 0 = index order, -1 = the site-order "
+                         "tuner's choice) of the measured lattice")
+    ap.add_argument("--big-order", type=int, default=0,
+                    help="site order of the 160^3 HBM-resident launch (-1: tuned)")
     ap.add_argument("--grid", default="literal", choices=["literal", "straddle"],
                     help="replicas96: the configuration's beta_2 grid (broken phase "
                          "throughout) or one that straddles the transition")
@@ -870,6 +885,11 @@ def main():
         tuned = {}
     else:
         tuned = L.autotune(n_or=args.n_or, trials=10)
+    if args.order < 0:
+        tuned_order = L.autotune_site_order(n_or=args.n_or, trials=10)
+    else:
+        L.set_site_order(args.order)
+        tuned_order = {}
     L.sync()
 
     # timed region: K sweeps, synchronised on both sides
@@ -915,9 +935,11 @@ def main():
             "lattice": list(shape),
             "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
             "link_updates_per_s": 3 * V * (1 + args.n_or) / (ms * 1e-3),
-            "launch_shape": {"threads_per_block": L.get_shape()[0], "xcd_order": L.get_shape()[1]},
+            "launch_shape": {"threads_per_block": L.get_shape()[0], "xcd_order": L.get_shape()[1],
+                             "site_order": L.get_site_order()},
             "autotune_ms_per_sweep": {"%d%s" % (t, "x" if x else ""): v
                                       for (t, x), v in tuned.items()},
+            "site_order_ms_per_sweep": tuned_order,
         },
         "roofline": {
             "bound": "hbm",
@@ -946,7 +968,8 @@ def main():
                                    "higgs_accept": hist[-1, 8], "delta": d},
     }
     big = (160, 160, 160)
-    rec["roofline"]["hbm_resident"] = hbm_resident_or(big, args.n_or, L.get_shape())
+    rec["roofline"]["hbm_resident"] = hbm_resident_or(big, args.n_or, L.get_shape(),
+                                                      order=args.big_order)
     rec["roofline"]["hbm_resident"]["frac_of_1GiB_copy"] = (
         rec["roofline"]["hbm_resident"]["achieved"]
         / rec["roofline"]["copy_ceiling"]["hbm_resident_1GiB_GBps"])

@@ -88,6 +88,20 @@ int lceq_get_shape(lceq_lattice *L, int *threads_per_block, int *xcd_order);
  * counter and the history are restored.  ms (or NULL) receives 8 times per
  * sweep, shapes (64,128,256,512) x (launch order, XCD order).  Philox only. */
 int lceq_autotune(lceq_lattice *L, int n_or, int trials, double *ms);
+/* site order of the link and Higgs updates (no counterpart in the
+ * reference): strip_rows = 0 takes the sites in index order; a power of two
+ * >= 2 dividing ly takes the rows (y, z) of a parity in strips of that many y
+ * rows, each strip advancing through z, so that the z neighbours of the sites
+ * in flight stay in the L2 of a lattice larger than the Infinity Cache.
+ * Fields do not depend on the order; plaquette and Higgs sums are added in an
+ * order-dependent way.  Default 0. */
+int lceq_set_site_order(lceq_lattice *L, int strip_rows);
+int lceq_get_site_order(lceq_lattice *L, int *strip_rows);
+/* time `trials` production sweeps with site orders 0, 4, 8, 16 (those that
+ * divide ly) at the current launch shape, keep the fastest; fields,
+ * observables, the pass counter and the history are restored.  ms (or NULL)
+ * receives 4 times per sweep (-1 where an order does not fit). */
+int lceq_autotune_site_order(lceq_lattice *L, int n_or, int trials, double *ms);
 /* Measurement aid with no counterpart in the reference: a plain device copy
  * of a `bytes` buffer (a multiple of 16; one 16-byte element per thread),
  * 3 untimed copies, then `reps` copies timed with HIP events.  us_per_copy

@@ -4,6 +4,9 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblceq.so")
+# test-only build: no contraction, 1/sqrt and cos(2 pi r) as the CPU
+# restatement writes them (lceqcd_amd/build.py)
+ORACLE_FORMS_PATH = os.path.join(HERE, "liblceq_oracleforms.so")
 
 D, I, I64, U64, VP = C.c_double, C.c_int, C.c_int64, C.c_uint64, C.c_void_p
 DP, IP = C.POINTER(C.c_double), C.POINTER(C.c_int)
@@ -19,6 +22,9 @@ SIGNATURES = {
     "lceq_set_shape": [VP, I, I],
     "lceq_get_shape": [VP, IP, IP],
     "lceq_autotune": [VP, I, I, DP],
+    "lceq_set_site_order": [VP, I],
+    "lceq_get_site_order": [VP, IP],
+    "lceq_autotune_site_order": [VP, I, I, DP],
     "lceq_copy_bandwidth": [I64, I, DP],
     "lceq_rng_philox": [VP, U64, U64],
     "lceq_rng_feed": [VP, DP, I64],
@@ -115,28 +121,45 @@ class LceqError(RuntimeError):
     pass
 
 
+def _open(path):
+    if not os.path.exists(path):
+        raise LceqError(
+            "%s is not built; run `python -m lceqcd_amd.build` "
+            "(there is no CPU fallback)" % os.path.basename(path))
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = I
+    lib.lceq_last_error.argtypes = []
+    lib.lceq_last_error.restype = C.c_char_p
+    lib.lceq_measinit_key.argtypes = [IP]
+    lib.lceq_measinit_key.restype = U64
+    return lib
+
+
 def load():
     """Load the in-tree liblceq.so; fails loudly when it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise LceqError(
-                "liblceq.so is not built; run `python -m lceqcd_amd.build` "
-                "(there is no CPU fallback)")
-        lib = C.CDLL(LIB_PATH)
-        for name, args in SIGNATURES.items():
-            f = getattr(lib, name)
-            f.argtypes = args
-            f.restype = I
-        lib.lceq_last_error.argtypes = []
-        lib.lceq_last_error.restype = C.c_char_p
-        lib.lceq_measinit_key.argtypes = [IP]
-        lib.lceq_measinit_key.restype = U64
-        _lib = lib
+        _lib = _open(LIB_PATH)
     return _lib
 
 
-def check(rc):
+_oracle_forms = None
+
+
+def load_oracle_forms():
+    """The test-only build liblceq_oracleforms.so (same entry points; every
+    operation rounded as written, 1/sqrt and cos(2 pi r) in the CPU
+    restatement's forms).  Pass it as ``Lattice(..., lib=...)``."""
+    global _oracle_forms
+    if _oracle_forms is None:
+        _oracle_forms = _open(ORACLE_FORMS_PATH)
+    return _oracle_forms
+
+
+def check(rc, lib=None):
     if rc != 0:
-        msg = load().lceq_last_error().decode(errors="replace")
+        msg = (lib or load()).lceq_last_error().decode(errors="replace")
         raise LceqError("liblceq status %d: %s" % (rc, msg))

@@ -1,4 +1,5 @@
-"""Build liblceq.so in-tree with hipcc for gfx950.
+"""Build liblceq.so in-tree with hipcc for gfx950, and beside it the
+test-only liblceq_oracleforms.so.
 
     python -m lceqcd_amd.build
 """
@@ -12,19 +13,27 @@ SRC = [os.path.join(HERE, "csrc", f) for f in ("lceq_kernels.hip", "lceq_api.hip
                                                    "lceq_ranlux.hip", "lceq_conf.hip",
                                                    "lceq_mlp.hip", "lceq_static.hip")]
 OUT = os.path.join(HERE, "liblceq.so")
+# Test-only build of the same sources: every operation rounded as written
+# (no contraction) and 1/sqrt, cos(2 pi r) in the CPU restatement's forms
+# (LCEQ_ORACLE_FORMS), for chains compared with the restatement bit for bit.
+# Nothing in the package loads it unless asked (_lib.load_oracle_forms).
+OUT_ORACLE_FORMS = os.path.join(HERE, "liblceq_oracleforms.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # keep the compiler away from packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # a * b + c may become one fused multiply-add (v_fma_f64, one rounding):
-    # 5% faster sweeps than with every operation rounded as written, at a
-    # difference from the CPU restatement (no FMA, as the reference's x86
-    # build) of a few ulp per update
-    "-ffp-contract=fast",
     "-Wall", "-Wno-unknown-warning-option",
 ]
+# a * b + c inside one expression may become one fused multiply-add
+# (v_fma_f64, one rounding), at a difference from the CPU restatement (no FMA,
+# as the reference's x86 build) of a few ulp per update.  Products are never
+# fused across statements (=on, not =fast): the rounding of every expression
+# is then fixed by the source, whatever kernel the site code is inlined into,
+# so the batch kernels and the single-lattice kernels give the same bits.
+PRODUCT_FLAGS = ["-ffp-contract=on"]
+ORACLE_FORMS_FLAGS = ["-ffp-contract=off", "-DLCEQ_ORACLE_FORMS"]
 
 
 FORTRAN = os.path.join(HERE, "fortran")
@@ -47,16 +56,27 @@ def build_fortran():
 
 
 def build(verbose=False):
-    """Compile every HIP source into lceqcd_amd/liblceq.so (always rebuilds)."""
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + SRC + ["-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building liblceq.so")
-    if verbose:
-        sys.stderr.write(r.stderr)
-    os.replace(tmp, OUT)
+    """Compile every HIP source into lceqcd_amd/liblceq.so and the test-only
+    liblceq_oracleforms.so (always rebuilds; the two compiles run side by
+    side)."""
+    jobs = []
+    for out, extra in ((OUT, PRODUCT_FLAGS), (OUT_ORACLE_FORMS, ORACLE_FORMS_FLAGS)):
+        tmp = out + ".tmp"
+        cmd = [HIPCC] + FLAGS + extra + SRC + ["-o", tmp]
+        jobs.append((out, tmp, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.PIPE, text=True)))
+    failed = []
+    for out, tmp, proc in jobs:
+        so, se = proc.communicate()
+        if proc.returncode != 0:
+            sys.stderr.write(so + se)
+            failed.append(os.path.basename(out))
+            continue
+        if verbose:
+            sys.stderr.write(se)
+        os.replace(tmp, out)
+    if failed:
+        raise RuntimeError("hipcc failed building " + ", ".join(failed))
     if os.path.exists(FC):
         build_fortran()
     return OUT

@@ -283,6 +283,84 @@ extern "C" int lceq_autotune(lceq_lattice* L, int n_or, int trials, double* ms) 
   return rc;
 }
 
+extern "C" int lceq_set_site_order(lceq_lattice* L, int strip_rows) {
+  NEEDL(L);
+  int sy = strip_rows;
+  if (sy != 0 && (sy < 2 || (sy & (sy - 1)) || L->g.ly % sy))
+    return fail(LCEQ_ERR_ARG, "strip rows must be 0 or a power of two >= 2 dividing ly = %d",
+                L->g.ly);
+  HIPCHK(hipStreamSynchronize(L->stream));
+  L->strip = sy;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_get_site_order(lceq_lattice* L, int* strip_rows) {
+  NEEDL(L);
+  if (strip_rows) *strip_rows = L->strip;
+  return LCEQ_OK;
+}
+
+// Time `trials` production sweeps with each site order of 0, 4, 8 and 16
+// strip rows (those dividing ly) at the current launch shape and keep the
+// fastest; the fields, the observables, the pass counter and the history are
+// restored.  ms (or NULL) receives 4 times per sweep, -1 for an order that
+// does not fit the lattice.
+extern "C" int lceq_autotune_site_order(lceq_lattice* L, int n_or, int trials, double* ms) {
+  NEEDL(L);
+  if (trials < 1 || n_or < 0) return fail(LCEQ_ERR_ARG, "bad autotune arguments");
+  if (L->rng_mode != LCEQ_RNG_PHILOX) return fail(LCEQ_ERR_ARG, "autotune needs Philox streams");
+  size_t np = (size_t)L->g.npad;
+  std::vector<double> u(24 * np), a(6 * np), o(LCEQ_OBS_N);
+  HIPCHK(hipMemcpyAsync(u.data(), L->U, u.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipMemcpyAsync(a.data(), L->A, a.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipMemcpyAsync(o.data(), L->obs, o.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  uint64_t pass0 = L->pass;
+  int rows0 = L->hist_rows;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(LCEQ_ERR_HIP, "hipEventCreate failed");
+  }
+  const int orders[4] = {0, 4, 8, 16};
+  double best = 1e300, delta = o[OBS_DELTA];
+  int best_sy = L->strip, rc = LCEQ_OK;
+  for (int i = 0; i < 4 && rc == LCEQ_OK; ++i) {
+    int sy = orders[i];
+    if (sy && L->g.ly % sy) {
+      if (ms) ms[i] = -1.0;
+      continue;
+    }
+    L->strip = sy;
+    if ((rc = lceq_sweep(L, LCEQ_SWEEP_KP_OR_METRO, 1, n_or, &delta, 0))) break;
+    (void)hipEventRecord(e0, L->stream);
+    if ((rc = lceq_sweep(L, LCEQ_SWEEP_KP_OR_METRO, trials, n_or, &delta, 0))) break;
+    (void)hipEventRecord(e1, L->stream);
+    float el = 0.0f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+      rc = fail(LCEQ_ERR_HIP, "timing failed");
+      break;
+    }
+    double per = (double)el / trials;
+    if (ms) ms[i] = per;
+    if (per < best) {
+      best = per;
+      best_sy = sy;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  L->strip = best_sy;
+  L->pass = pass0;
+  L->hist_rows = rows0;
+  HIPCHK(hipMemcpyAsync(L->U, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipMemcpyAsync(L->A, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipMemcpyAsync(L->obs, o.data(), o.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+  HIPCHK(hipStreamSynchronize(L->stream));
+  return rc;
+}
+
 extern "C" int lceq_set_imode(lceq_lattice* L, int imode) {
   NEEDL(L);
   if (imode != 0 && imode != 1) return fail(LCEQ_ERR_ARG, "imode must be 0 or 1");

@@ -7,6 +7,37 @@
 
 namespace lceq {
 
+// 1/sqrt(t) and cos, sin(2 pi r) in the update kernels.  The product library
+// takes the reciprocal square root (within an ulp of 1/sqrt, one operation)
+// and cospi / sincospi of 2r (no argument reduction by a rounded 2 pi; the
+// reference's cos(tpi * r) differs by an ulp or so).  The test-only library
+// built with LCEQ_ORACLE_FORMS (and without contraction) rounds them as the
+// CPU restatement writes them, 1 / sqrt(t) and cos(tpi * r) with
+// tpi = 2 acos(-1), so that its chains differ from the restatement's only
+// where a libm function (log, cos, sin, exp) rounds differently.
+__device__ __forceinline__ double inv_sqrt(double t) {
+#ifdef LCEQ_ORACLE_FORMS
+  return 1.0 / sqrt(t);
+#else
+  return rsqrt(t);
+#endif
+}
+__device__ __forceinline__ double cos_2pi(double r) {
+#ifdef LCEQ_ORACLE_FORMS
+  return cos((2.0 * M_PI) * r);
+#else
+  return cospi(2.0 * r);
+#endif
+}
+__device__ __forceinline__ void sincos_2pi(double r, double* sn, double* cs) {
+#ifdef LCEQ_ORACLE_FORMS
+  *sn = sin((2.0 * M_PI) * r);
+  *cs = cos((2.0 * M_PI) * r);
+#else
+  sincospi(2.0 * r, sn, cs);
+#endif
+}
+
 // ---------------------------------------------------------------- complex
 
 struct cpx {
@@ -152,6 +183,28 @@ struct Coord {
   int x, y, z;
 };
 
+// Site order of an update launch.  sy = 0: the site index order (position P
+// = site s).  sy = 2^k > 0: the rows (y, z) of the parity in strips of sy
+// consecutive y, each strip advancing through z before the next strip; P
+// runs along a row (h1 sites), then over the sy rows of one z, then z.  The
+// rows of one z in a strip are contiguous in memory, and the z neighbours of
+// a position are sy * h1 positions away instead of h2, so the blocks of one
+// XCD reuse them from its L2 on lattices too large for the Infinity Cache.
+struct SiteOrder {
+  int sy, lsy;  // strip height and its log2
+  int rows;     // sy * lz: rows per strip
+  float rrows;  // 1 / rows
+};
+__host__ __device__ inline SiteOrder site_order(const Geom& g, int sy) {
+  SiteOrder o;
+  o.sy = sy;
+  o.lsy = 0;
+  while ((1 << o.lsy) < sy) ++o.lsy;
+  o.rows = sy * g.lz;
+  o.rrows = sy ? 1.0f / (float)o.rows : 0.0f;
+  return o;
+}
+
 __device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
   int z = div_small(s, g.h2, g.rh2);
   int r = s - z * g.h2;
@@ -159,6 +212,18 @@ __device__ __forceinline__ Coord site_coord(const Geom& g, int p, int s) {
   int xh = r - y * g.h1;
   return Coord{2 * xh + ((p + y + z) & 1), y, z};
 }
+// the site at position P of the order o (P < N)
+__device__ __forceinline__ unsigned order_site(const Geom& g, const SiteOrder& o, unsigned P) {
+  if (!o.sy) return P;
+  int r = div_small((int)P, g.h1, g.rh1);  // row in the order
+  int k = (int)P - r * g.h1;
+  int strip = div_small(r, o.rows, o.rrows);
+  int q = r - strip * o.rows;
+  int z = q >> o.lsy;
+  int y = (strip << o.lsy) + (q - (z << o.lsy));
+  return (unsigned)(k + g.h1 * y + g.h2 * z);
+}
+
 // Index changes of the moves +1 (p) and -1 (m) along x, y, z from a site at
 // c, periodic: a y or z move keeps x/2 and changes the index by +-h1 or
 // +-h2 (wrapping at the faces); an x move changes x/2 by 0 or +-1 with the

@@ -57,6 +57,8 @@ struct lceq_lattice {
   // launch shape of the link and Higgs updates: threads (= sites) per block
   // 64/128/256/512 and the XCD-aware block order (partials sized for 64)
   int tb, xcd;
+  // site order of those launches: strips of `strip` y rows (0: index order)
+  int strip;
   uint64_t seed, pass;
   // timing of one kernel class (bench instrumentation): one event pair per
   // run of consecutive launches of that class, launches counted inside runs

@@ -37,6 +37,7 @@ struct LinkArgs {
   int reduce;
   double* part;  // partial sums of this launch's slot: [LCEQ_NQ][pstride]
   int pstride, nb, xcd;
+  SiteOrder order;
 };
 
 // Block index in the launch's block order: XCD-aware (contiguous sites per
@@ -50,12 +51,12 @@ __device__ __forceinline__ int shape_block(int nb, int xcd) {
 template <bool KP, bool FED, int TBX, int MU>
 __global__ __launch_bounds__(TBX) void k_link(LinkArgs a) {
   int blk = shape_block(a.nb, a.xcd);
-  unsigned s = (unsigned)(blk * TBX + threadIdx.x);
+  unsigned P = (unsigned)(blk * TBX + threadIdx.x);
   double red[3] = {0.0, 0.0, 0.0};
-  if (s < (unsigned)a.site.g.n) {
+  if (P < (unsigned)a.site.g.n) {
     LinkSite p = a.site;
     p.mu = MU;
-    link_update_site<KP, FED>(p, s, red);
+    link_update_site<KP, FED>(p, order_site(p.g, a.order, P), red);
   }
   if (a.reduce) block_sum_store<3, TBX / 64>(red, a.part, a.pstride, blk);
 }
@@ -104,6 +105,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.pstride = L->pstride;
   a.nb = shape_blocks(L);
   a.xcd = L->xcd;
+  a.order = site_order(L->g, L->strip);
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
   prof_count(L, kc);
@@ -279,14 +281,16 @@ struct HiggsArgs {
   HiggsSite site;
   double* part;
   int pstride, nb, xcd;
+  SiteOrder order;
 };
 
 template <int KIND, bool FED, int TBX>
 __global__ __launch_bounds__(TBX) void k_higgs(HiggsArgs a) {
   int blk = shape_block(a.nb, a.xcd);
-  unsigned s = (unsigned)(blk * TBX + threadIdx.x);
+  unsigned P = (unsigned)(blk * TBX + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
-  if (s < (unsigned)a.site.g.n) higgs_update_site<KIND, FED>(a.site, s, red);
+  if (P < (unsigned)a.site.g.n)
+    higgs_update_site<KIND, FED>(a.site, order_site(a.site.g, a.order, P), red);
   block_sum_store<4, TBX / 64>(red, a.part, a.pstride, blk);
 }
 
@@ -321,6 +325,7 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.pstride = L->pstride;
   a.nb = shape_blocks(L);
   a.xcd = L->xcd;
+  a.order = site_order(L->g, L->strip);
   prof_begin(L, KC_HIGGS);
   prof_count(L, KC_HIGGS);
   if (kind == 1) {

@@ -108,7 +108,7 @@ __device__ __forceinline__ double hop_action(double beta_a, const vec3& a, const
 // of 1/sqrt, one operation instead of a square root and a division)
 __device__ __forceinline__ su2 normalized(su2 u) {
   double t = (u.a.re * u.a.re + u.a.im * u.a.im) + (u.b.re * u.b.re + u.b.im * u.b.im);
-  t = rsqrt(t);
+  t = inv_sqrt(t);
   return su2{c_scale(t, u.a), c_scale(t, u.b)};
 }
 
@@ -173,15 +173,15 @@ __device__ __forceinline__ void link_prepare(const LinkSite& p, unsigned s, Link
     double r1 = o.r1, r2 = o.r2;
     if (r1 == 0.0) r1 = 0.0000033;
     if (r2 == 0.0) r2 = 0.0000066;
-    // cos(2 pi r) as cospi(2 r): no argument reduction by a rounded 2 pi
-    // (the reference's cos(tpi*r3) differs by an ulp or so); 0.5 us less per
-    // KP launch at 64^3
-    double cc = cospi(2.0 * o.r3);
+    // cos(2 pi r) as cospi(2 r) in the product library (cos_2pi): no
+    // argument reduction by a rounded 2 pi (the reference's cos(tpi*r3)
+    // differs by an ulp or so); 0.5 us less per KP launch at 64^3
+    double cc = cos_2pi(o.r3);
     o.kp_q = log(r1) + log(r2) * (cc * cc);
     o.kp_a2 = 2.0 * (o.r5 - 0.5);
     double tt = 1.0 - o.kp_a2 * o.kp_a2;
     o.kp_st = tt >= 0.0 ? sqrt(tt) : -1.0;
-    sincospi(2.0 * o.r6, &o.kp_sin, &o.kp_cos);  // sin, cos(2 pi r6)
+    sincos_2pi(o.r6, &o.kp_sin, &o.kp_cos);  // sin, cos(2 pi r6)
   }
 }
 
@@ -196,7 +196,7 @@ __device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const
   su2 V = staple_sum(o.k);
   // f/agauge.f:71-84
   double t = (V.a.re * V.a.re + V.a.im * V.a.im) + (V.b.re * V.b.re + V.b.im * V.b.im);
-  t = rsqrt(t);
+  t = inv_sqrt(t);
   su2 vh{c_scale(t, V.a), c_scale(t, V.b)};
   su2 h;
   if (KP) {
@@ -391,7 +391,7 @@ __device__ __forceinline__ void higgs_update_site(const HiggsSite& p, unsigned s
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         double w1 = -1.0 * (1.0 / width) * log(w[2 * i]);
-        double gi = sqrt(w1) * cospi(2.0 * w[2 * i + 1]);  // cos(2 pi w), as in the KP step
+        double gi = sqrt(w1) * cos_2pi(w[2 * i + 1]);  // cos(2 pi w), as in the KP step
         an.x[i] = gi + p.beta_a * F.x[i] * cx;
       }
     } else {  // f/ahiggs3.f:83-87

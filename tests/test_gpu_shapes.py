@@ -57,3 +57,47 @@ def test_copy_bandwidth_probe():
     rate = 2 * nbytes / (us.value * 1e-6) / 1e9
     print("copy probe GB/s", rate, flush=True)
     assert np.isfinite(us.value) and 0.0 < rate < 20000.0
+
+
+def _run_order(order, lat, tune=False):
+    import lceqcd_amd
+    L = lceqcd_amd.Lattice(*lat)
+    L.set_couplings(*COUP)
+    L.hotstart(6, 0.5)
+    L.rng_philox(6, 0)
+    d = L.sweep(0, 2, n_or=2, delta=0.4, measflag=0)
+    L.set_site_order(order)
+    ms = L.autotune_site_order(n_or=2, trials=2) if tune else None
+    d = L.sweep(0, 3, n_or=2, delta=d, measflag=1)
+    out = L.get_links(), L.get_higgs(), L.history(3), d, L.get_site_order(), ms
+    L.close()
+    return out
+
+
+@pytest.mark.parametrize("lat,order", [((16, 16, 16), 4), ((16, 16, 16), 16), ((10, 8, 12), 2),
+                                       ((8, 32, 6), 8)])
+def test_site_orders_give_the_same_fields(lat, order):
+    """Strips of y rows advancing through z (lceq_set_site_order) visit every
+    site of a parity once: the same fields bit for bit as the index order."""
+    print("site order", lat, order, flush=True)
+    u0, a0, h0, d0, _, _ = _run_order(0, lat)
+    u1, a1, h1, d1, got, _ = _run_order(order, lat)
+    assert got == order
+    assert np.array_equal(u0, u1) and np.array_equal(a0, a1)
+    assert d0 == d1
+    np.testing.assert_allclose(h1, h0, rtol=1e-13, atol=1e-15)
+
+
+def test_site_order_tuner_restores_the_chain():
+    import lceqcd_amd
+    lat = (16, 16, 16)
+    u0, a0, h0, d0, _, _ = _run_order(0, lat)
+    u1, a1, h1, d1, got, ms = _run_order(0, lat, tune=True)
+    print("site order tuner", ms, "->", got, flush=True)
+    assert set(ms) == {0, 4, 8, 16} and ms[got] == min(ms.values())
+    assert np.array_equal(u0, u1) and np.array_equal(a0, a1)
+    L = lceqcd_amd.Lattice(16, 16, 16)
+    with pytest.raises(lceqcd_amd.LceqError, match="strip rows"):
+        L.set_site_order(3)
+    with pytest.raises(lceqcd_amd.LceqError, match="strip rows"):
+        L.set_site_order(32)
