@@ -4,9 +4,9 @@ vectors and against the CPU restatement, on the same inputs.
 Bar: 1e-12 relative in FP64 for fields and observables (tolerances are stated
 per case in tests/parity_cases.py), exact for counts, zero accept/reject flips
 (a flip shows up as an O(1) field difference at a site and is counted).  The
-library fuses multiply-adds where the compiler can; the CPU restatement rounds
-every operation as the reference's x86 build does, so fields differ by a few
-ulp per update."""
+library fuses a multiply into an add written in the same expression; the CPU
+restatement rounds every operation as the reference's x86 build does, so
+fields differ by a few ulp per update."""
 import numpy as np
 import pytest
 
@@ -144,9 +144,10 @@ def test_philox_production_sweeps_16(make, omake):
     1.6e-13 after one sweep, 1.7e-12 after two and 9e-11 after six
     (tests/test_oracle_chaos.py).  That envelope is computed here on the same
     configuration; the GPU chain (fused multiply-adds, ocml, other but equal
-    algebraic forms) must stay within 100 times it after every sweep, within
-    1e-12 after one sweep, 1e-11 after two and 1e-9 after six, and make every
-    accept/reject decision as the oracle does.  The 1e-12 bar for one update
+    algebraic forms) must stay within twice it after every sweep (measured
+    0.2-0.7 times it), within 1e-12 after one sweep, 1e-11 after two and
+    1e-10 after six, and make every accept/reject decision as the oracle
+    does.  The 1e-12 bar for one update
     on identical inputs is checked in test_philox_link_updates and
     test_philox_or_and_metropolis_flips_counted."""
     shape = (16, 16, 16)
@@ -161,7 +162,7 @@ def test_philox_production_sweeps_16(make, omake):
     P.put_higgs(O.get_higgs())
     P.rng_philox(1003, 7)
     P.set_renorm_on_write(1)
-    bars = {1: 1e-12, 2: 1e-11, 6: 1e-9}
+    bars = {1: 1e-12, 2: 1e-11, 6: 1e-10}
     d = dp = dG = 0.4
     for it in range(1, 7):
         for X in (O, P):
@@ -180,7 +181,7 @@ def test_philox_production_sweeps_16(make, omake):
               "ratio %.1f" % (it, dev_u, dev_a, env, max(dev_u, dev_a) / max(env, 1e-300)),
               flush=True)
         assert flips_u == 0 and flips_a == 0
-        assert max(dev_u, dev_a) <= 100.0 * env
+        assert max(dev_u, dev_a) <= 2.0 * env
         if it in bars:
             assert_close(G.get_links(), O.get_links(), bars[it], "links")
             assert_close(G.get_higgs(), O.get_higgs(), bars[it], "higgs")
