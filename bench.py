@@ -382,6 +382,59 @@ def cpu_baseline_reference_slabs(L, seconds, procs):
                       "oracle/_ref/refharness64_64" % ((procs,) + shape + L.shape + (k, procs, tk))}
 
 
+MPI_GRIDS = {1: (1, 1, 1), 2: (1, 1, 2), 4: (1, 2, 2), 8: (2, 2, 2), 16: (2, 2, 4)}
+
+
+def cpu_baseline_reference_mpi(shape, procs_list, seconds):
+    """The reference's own sweep under MPI with its halo exchanges
+    (oracle/_ref/refmpi32_64: its ahiggs2, ahiggs3, agauge, renorm and
+    typeconstruct, adresses as shipped, 4-byte reals, the rank grid of
+    f/topology.f), run with mpiexec on 1..16 host cores; about `seconds` of
+    sweeps per rank count.  Returns one record per rank count, or None when
+    the build or mpiexec is absent."""
+    import resource
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "refmpi32_64")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.access(exe, os.X_OK) and os.access(mpiexec, os.X_OK)):
+        return None
+
+    def stack():
+        hard = resource.getrlimit(resource.RLIMIT_STACK)[1]
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    def run(procs, nwarm, nsweep):
+        grid = MPI_GRIDS[procs]
+        cmd = [mpiexec, "-n", str(procs), exe] + [str(v) for v in shape + grid] + [
+            str(nwarm), str(nsweep)]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                           timeout=600, preexec_fn=stack, cwd=ROOT)
+        line = [x for x in r.stdout.splitlines() if x.startswith("refmpi")]
+        if r.returncode != 0 or not line:
+            raise RuntimeError("refmpi failed: %s %s" % (r.stdout[-500:], r.stderr[-500:]))
+        f = line[-1].split()
+        return float(f[9]), float(f[10])
+
+    V = shape[0] * shape[1] * shape[2]
+    out = []
+    for procs in procs_list:
+        grid = MPI_GRIDS[procs]
+        if any(n % (2 * g) for n, g in zip(shape, grid)):
+            continue
+        t1, _ = run(procs, 1, 2)
+        k = max(3, min(2000, int(seconds / max(t1, 1e-3))))
+        tk, plaq = run(procs, 2, k)
+        out.append({"value": V / tk, "unit": "site-updates/s", "cores": procs,
+                    "kind": "reference", "rank_grid": list(grid), "seconds_per_sweep": tk,
+                    "plaq0": plaq,
+                    "sample": "mpiexec -n %d oracle/_ref/refmpi32_64: the reference's own "
+                              "f/main.f:37-41 sweep (ahiggs2, ahiggs3, agauge, renorm, 4-byte "
+                              "reals as shipped) on %dx%dx%d over a %dx%dx%d rank grid with its "
+                              "halo exchanges, %d timed sweeps, slowest rank"
+                              % ((procs,) + tuple(shape) + tuple(grid) + (k,))})
+    return out
+
+
 def refsched64(args):
     """The reference schedule, gauge fixing and correlators at 64^3."""
     import lceqcd_amd
@@ -482,12 +535,19 @@ def refsched64(args):
     if not args.no_cpu:
         cb = cpu_baseline_reference(L, V, args.cpu_seconds)
         cm = cpu_baseline_reference_slabs(L, args.cpu_seconds, args.cpu_procs)
-        if cm is not None:
+        mpi = cpu_baseline_reference_mpi(shape, [p for p in (1, 2, 4, 8, 16)
+                                                 if p <= args.cpu_procs], args.cpu_seconds / 4)
+        if mpi:
+            rec["cpu_baseline"] = mpi[-1]
+            rec["config"]["reference_mpi_scaling"] = mpi
+            if cm is not None:
+                rec["config"]["reference_slabs_upper_bound"] = cm
+        elif cm is not None:
             rec["cpu_baseline"] = cm
-            if cb is not None:
-                rec["config"]["reference_one_rank"] = cb
         elif cb is not None:
             rec["cpu_baseline"] = cb
+        if cb is not None:
+            rec["config"]["reference_one_rank"] = cb
     print(json.dumps(rec), flush=True)
 
 
