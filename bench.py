@@ -93,6 +93,7 @@ import argparse
 import json
 import os
 import sys
+import subprocess
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -535,8 +536,13 @@ def refsched64(args):
     if not args.no_cpu:
         cb = cpu_baseline_reference(L, V, args.cpu_seconds)
         cm = cpu_baseline_reference_slabs(L, args.cpu_seconds, args.cpu_procs)
-        mpi = cpu_baseline_reference_mpi(shape, [p for p in (1, 2, 4, 8, 16)
-                                                 if p <= args.cpu_procs], args.cpu_seconds / 4)
+        try:
+            mpi = cpu_baseline_reference_mpi(shape, [p for p in (1, 2, 4, 8, 16)
+                                                     if p <= args.cpu_procs],
+                                             args.cpu_seconds / 4)
+        except (RuntimeError, OSError, subprocess.SubprocessError) as e:
+            mpi = None
+            rec["config"]["reference_mpi_error"] = str(e)[:500]
         if mpi:
             rec["cpu_baseline"] = mpi[-1]
             rec["config"]["reference_mpi_scaling"] = mpi
