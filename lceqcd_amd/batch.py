@@ -56,9 +56,17 @@ class Batch:
         a, p = self._u64(seeds)
         check(self._L.lceq_batch_hotstart(self.h, p, sigma))
 
+    def _field(self, r, arr, ncomp):
+        n = self.sites[r] // 2
+        shape = (3, 2, n, 4) if ncomp == 4 else (2, n, 3)
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError("replica %d: expected shape %s, got %s" % (r, shape, a.shape))
+        return a, n
+
     def put_links(self, r, u):
-        u = np.ascontiguousarray(u, dtype=np.float64)
-        check(self._L.lceq_batch_put_links(self.h, r, _dp(u), self.sites[r] // 2))
+        u, n = self._field(r, u, 4)
+        check(self._L.lceq_batch_put_links(self.h, r, _dp(u), n))
 
     def get_links(self, r):
         n = self.sites[r] // 2
@@ -67,8 +75,8 @@ class Batch:
         return u
 
     def put_higgs(self, r, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        check(self._L.lceq_batch_put_higgs(self.h, r, _dp(a), self.sites[r] // 2))
+        a, n = self._field(r, a, 3)
+        check(self._L.lceq_batch_put_higgs(self.h, r, _dp(a), n))
 
     def get_higgs(self, r):
         n = self.sites[r] // 2
