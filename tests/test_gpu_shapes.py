@@ -69,7 +69,10 @@ def _run_order(order, lat, tune=False):
     L.set_site_order(order)
     ms = L.autotune_site_order(n_or=2, trials=2) if tune else None
     d = L.sweep(0, 3, n_or=2, delta=d, measflag=1)
-    out = L.get_links(), L.get_higgs(), L.history(3), d, L.get_site_order(), ms
+    h = L.history(3)
+    L.sweep(1, 2, measflag=1)  # the reference schedule: ahiggs2, ahiggs3, agauge, renorm
+    h = np.concatenate([h, L.history(2)])
+    out = L.get_links(), L.get_higgs(), h, d, L.get_site_order(), ms
     L.close()
     return out
 
@@ -78,7 +81,9 @@ def _run_order(order, lat, tune=False):
                                        ((8, 32, 6), 8)])
 def test_site_orders_give_the_same_fields(lat, order):
     """Strips of y rows advancing through z (lceq_set_site_order) visit every
-    site of a parity once: the same fields bit for bit as the index order."""
+    site of a parity once: the same fields bit for bit as the index order,
+    through production sweeps (KP, overrelaxation, ahiggs1) and reference
+    sweeps (ahiggs2, ahiggs3, KP, renorm)."""
     print("site order", lat, order, flush=True)
     u0, a0, h0, d0, _, _ = _run_order(0, lat)
     u1, a1, h1, d1, got, _ = _run_order(order, lat)
