@@ -50,7 +50,8 @@ def test_strip_site_order_is_a_permutation_with_local_z_neighbours():
     launch one to one onto the sites of the parity (so every load and store
     stays in bounds and every site is updated once), keeps a row's x run
     contiguous, and puts a site's z neighbour sy rows later in the order."""
-    for shape in ((4, 4, 4), (8, 4, 6), (64, 64, 64), (160, 160, 160), (32, 32, 128), (24, 48, 8)):
+    for shape in ((2, 2, 2), (2, 8, 4), (4, 4, 4), (8, 4, 6), (64, 64, 64), (160, 160, 160),
+                  (32, 32, 128), (24, 48, 8), (6, 16, 2)):
         lx, ly, lz = shape
         n = lx * ly * lz // 2
         P = np.arange(n, dtype=np.int64)
