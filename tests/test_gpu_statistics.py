@@ -14,8 +14,8 @@ Errors: Madras-Sokal integrated autocorrelation times for the means; a
 delete-one-block jackknife (30 blocks) for the connected correlator, with the
 ensemble mean subtracted, and for the cosh effective mass between
 separations 1 and 2.  The seeds are fixed, so the outcome is deterministic;
-every z-score is printed.  The bar is |z| < 2 for each quantity, the
-north-star criterion (measured: plaquette -1.76, rho +1.16, Polyakov loop
+every z-score is printed.  The bar is |z| < 2 for each quantity: independent
+chains agree within 2 sigma (measured: plaquette -1.76, rho +1.16, Polyakov loop
 +0.24, C(0..3) -1.69, +0.63, +0.45, +1.07, screening mass -0.71).  From two
 independent hot starts the plaquettes would differ by about 9 sigma on their
 own (tests/test_oracle_chaos.py), which is why both chains start from one
