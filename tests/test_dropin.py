@@ -126,7 +126,7 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     program, here through the drop-in layer on 24^3.  Every drop-in routine
     copies the field it changed back into COMMON, and in the default mode it
     also draws the reference's uniforms on the host and uploads them.  Two
-    runs of 20 and 220 sweeps give the cost per sweep without the one-off
+    runs of 20 and 420 sweeps give the cost per sweep without the one-off
     device set-up.  In Philox mode (LCEQ_DROPIN_RNG=philox, streams seeded as
     f/measinit.f:214-217 seeds rank 0) the same sweeps of a device-resident
     lattice with the same key end on the same plaquette."""
@@ -140,12 +140,12 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     for mode in ("philox", "reference"):
         env = {"LCEQ_DROPIN_RNG": "philox"} if mode == "philox" else {}
         t = {}
-        for nrep in (20, 220):
+        for nrep in (20, 420):
             print("drop-in schedule", mode, nrep, flush=True)
             r = refrun.run("time", shape, u, a, nrep=nrep, exe=EXE, env=env, **coup)
             t[nrep] = float(r[0][0])
             plaq[mode, nrep] = float(r[1][0])
-        per_sweep[mode] = (t[220] - t[20]) / 200.0
+        per_sweep[mode] = (t[420] - t[20]) / 400.0
     L = lceqcd_amd.Lattice(*shape)
     L.set_couplings(coup["beta"], coup["beta_a"], coup["beta_2"], coup["beta_4"])
     L.set_imode(0)
@@ -156,14 +156,14 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     L.history(20)
     L.sync()
     t0 = time.perf_counter()
-    L.sweep(lceqcd_amd.SWEEP_REFERENCE, 200, measflag=1)
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, 400, measflag=1)
     L.sync()
-    dev = (time.perf_counter() - t0) / 200.0
-    p_dev = float(L.history(200)[-1, 0])
+    dev = (time.perf_counter() - t0) / 400.0
+    p_dev = float(L.history(400)[-1, 0])
     L.close()
     print("24^3 reference schedule, ms per sweep: drop-in %.3f (Philox), %.3f (reference "
           "uniforms), device-resident %.3f; plaquette drop-in %.15f device %.15f"
           % (1e3 * per_sweep["philox"], 1e3 * per_sweep["reference"], 1e3 * dev,
-             plaq["philox", 220], p_dev), flush=True)
-    assert abs(p_dev - plaq["philox", 220]) <= 1e-12 * abs(p_dev)
+             plaq["philox", 420], p_dev), flush=True)
+    assert abs(p_dev - plaq["philox", 420]) <= 1e-12 * abs(p_dev)
     assert 0.0 < dev and per_sweep["philox"] > 0.0 and per_sweep["reference"] > 0.0
