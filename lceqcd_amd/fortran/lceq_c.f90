@@ -15,6 +15,10 @@ module lceq_c
 
   ! lattice handle of the drop-in routines (lceq_dropin.f)
   type(c_ptr), save :: lceq_handle = c_null_ptr
+  ! drop-in coherence (lceq_dropin.f): with LCEQ_DROPIN_SYNC=deferred the
+  ! update routines leave u and a on the device, and lceq_dropin_download
+  ! copies them into COMMON when host code needs them
+  logical, save :: lceq_deferred = .false.
 
   interface
     integer(c_int) function lceq_create(lx, ly, lz, h) bind(C, name="lceq_create")

@@ -25,7 +25,11 @@
 *     the device lattice.  Every routine that changes a field copies it
 *     back into COMMON before returning, so unmodified host code (meas2a,
 *     confsave) reads current values.  Code that changes u or a in COMMON
-*     between calls calls lceq_dropin_upload afterwards.
+*     between calls calls lceq_dropin_upload afterwards.  With
+*     LCEQ_DROPIN_SYNC=deferred the update routines leave u and a on the
+*     device (the copies back cost more than the updates); host code then
+*     calls lceq_dropin_download before it reads u or a (before confsave,
+*     for instance).  The gauge-fixed u_f, a_f are copied back either way.
 
 *     create the device lattice from the COMMON blocks and upload u, a
       subroutine lceq_dropin_attach
@@ -48,6 +52,8 @@
      &                 lceq_handle)
       call lceq_check(rc, 'lceq_create')
       call lceq_dropin_params
+      call get_environment_variable('LCEQ_DROPIN_SYNC', mode, ln, st)
+      lceq_deferred = st .eq. 0 .and. mode(1:8) .eq. 'deferred'
       call get_environment_variable('LCEQ_DROPIN_RNG', mode, ln, st)
       if (st .eq. 0 .and. mode(1:6) .eq. 'philox') then
         call lceq_check(lceq_rng_measinit(lceq_handle, 0, ls),
@@ -104,11 +110,13 @@
       call lceq_check(lceq_put_higgs(lceq_handle, a, ld), 'put_higgs')
       end
 
+*     the updated u (a) back into COMMON, unless the copies are deferred
       subroutine lceq_dropin_get_links
       use lceq_c
       implicit none
       INCLUDE 'parameter.inc'
       INCLUDE 'gaugefield.inc'
+      if (lceq_deferred) return
       call lceq_check(lceq_get_links(lceq_handle, u,
      &     int(maxarray, c_int64_t)), 'get_links')
       end
@@ -118,6 +126,21 @@
       implicit none
       INCLUDE 'parameter.inc'
       INCLUDE 'gaugefield.inc'
+      if (lceq_deferred) return
+      call lceq_check(lceq_get_higgs(lceq_handle, a,
+     &     int(maxarray, c_int64_t)), 'get_higgs')
+      end
+
+*     u and a from the device into COMMON (needed with
+*     LCEQ_DROPIN_SYNC=deferred before host code reads them)
+      subroutine lceq_dropin_download
+      use lceq_c
+      implicit none
+      INCLUDE 'parameter.inc'
+      INCLUDE 'gaugefield.inc'
+      if (.not. c_associated(lceq_handle)) return
+      call lceq_check(lceq_get_links(lceq_handle, u,
+     &     int(maxarray, c_int64_t)), 'get_links')
       call lceq_check(lceq_get_higgs(lceq_handle, a,
      &     int(maxarray, c_int64_t)), 'get_higgs')
       end

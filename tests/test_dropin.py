@@ -129,7 +129,9 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     runs of 20 and 420 sweeps give the cost per sweep without the one-off
     device set-up.  In Philox mode (LCEQ_DROPIN_RNG=philox, streams seeded as
     f/measinit.f:214-217 seeds rank 0) the same sweeps of a device-resident
-    lattice with the same key end on the same plaquette."""
+    lattice with the same key end on the same plaquette, with the copies back
+    made after every routine and with them deferred (LCEQ_DROPIN_SYNC=
+    deferred: the fields stay on the device until lceq_dropin_download)."""
     import time
     import lceqcd_amd
     from oracle import refrun
@@ -137,8 +139,11 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     u, a = rand_conf(shape, 11, 0.5)
     coup = dict(beta=9.0, beta_a=0.175, beta_2=-0.45, beta_4=0.025)
     per_sweep, plaq = {}, {}
-    for mode in ("philox", "reference"):
-        env = {"LCEQ_DROPIN_RNG": "philox"} if mode == "philox" else {}
+    envs = {"philox": {"LCEQ_DROPIN_RNG": "philox"},
+            "deferred": {"LCEQ_DROPIN_RNG": "philox", "LCEQ_DROPIN_SYNC": "deferred"},
+            "reference": {}}
+    for mode in ("philox", "deferred", "reference"):
+        env = envs[mode]
         t = {}
         for nrep in (20, 420):
             print("drop-in schedule", mode, nrep, flush=True)
@@ -161,9 +166,12 @@ def test_dropin_schedule_timed_against_the_device_resident_path():
     dev = (time.perf_counter() - t0) / 400.0
     p_dev = float(L.history(400)[-1, 0])
     L.close()
-    print("24^3 reference schedule, ms per sweep: drop-in %.3f (Philox), %.3f (reference "
-          "uniforms), device-resident %.3f; plaquette drop-in %.15f device %.15f"
-          % (1e3 * per_sweep["philox"], 1e3 * per_sweep["reference"], 1e3 * dev,
-             plaq["philox", 420], p_dev), flush=True)
+    print("24^3 reference schedule, ms per sweep: drop-in %.3f (Philox), %.3f (Philox, "
+          "copies deferred), %.3f (reference uniforms), device-resident %.3f; plaquette "
+          "drop-in %.15f, deferred %.15f, device %.15f"
+          % (1e3 * per_sweep["philox"], 1e3 * per_sweep["deferred"],
+             1e3 * per_sweep["reference"], 1e3 * dev, plaq["philox", 420],
+             plaq["deferred", 420], p_dev), flush=True)
     assert abs(p_dev - plaq["philox", 420]) <= 1e-12 * abs(p_dev)
-    assert 0.0 < dev and per_sweep["philox"] > 0.0 and per_sweep["reference"] > 0.0
+    assert plaq["deferred", 420] == plaq["philox", 420]
+    assert 0.0 < dev and min(per_sweep.values()) > 0.0
