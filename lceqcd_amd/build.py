@@ -71,6 +71,8 @@ def build(verbose=False):
         if proc.returncode != 0:
             sys.stderr.write(so + se)
             failed.append(os.path.basename(out))
+            if os.path.exists(tmp):
+                os.remove(tmp)
             continue
         if verbose:
             sys.stderr.write(se)
