@@ -21,8 +21,9 @@ block, XCD-aware block order) from 10 sweeps per shape and restores the chain
 (--shape T,X fixes it instead, e.g. for profiling).
 Prints one JSON line.  value = site-updates/s (V sites per sweep).  The
 roofline object also carries copy_ceiling: the rate of the library's plain
-device copy (lceq_copy_bandwidth) moving the OR launch's own bytes, and of a
-1 GiB copy, timed in the same run; hbm_resident: the same OR launch timed on
+device copy (lceq_copy_bandwidth) moving the OR launch's own bytes, of a
+1 GiB copy, and of 30 read streams with one write stream
+(lceq_stream_bandwidth, 1 GB), timed in the same run; hbm_resident: the same OR launch timed on
 a 160^3 lattice (490 MB, more than the 256 MB Infinity Cache holds, which the
 64^3 field set does not exceed); and the source of its traffic figure.  The
 learned object compares the learned Higgs proposal in batched form with the
@@ -842,12 +843,26 @@ def copy_rate(nbytes, reps):
     return 2 * nbytes / (us.value * 1e-6) / 1e9
 
 
+def stream_rate(streams, elements, reps):
+    """GB/s of the library's multi-stream probe: `streams` arrays of
+    `elements` 16-byte values read at the same index, one written."""
+    import ctypes
+    from lceqcd_amd import _lib
+    us = ctypes.c_double()
+    _lib.check(_lib.load().lceq_stream_bandwidth(int(streams), int(elements), int(reps),
+                                                 ctypes.byref(us)))
+    return (streams + 1) * elements * 16 / (us.value * 1e-6) / 1e9
+
+
 def copy_ceiling(launch_bytes, achieved):
     """The practical bandwidth beside the 8 TB/s peak: a plain copy that
-    moves the launch's own bytes (half read, half written), and one of 1 GiB,
-    which no cache holds."""
+    moves the launch's own bytes (half read, half written), one of 1 GiB,
+    which no cache holds, and 30 read streams + 1 write stream of one 160^3
+    parity class each (1.0 GB, no cache holds it either), the access shape of
+    the link launch's ~30 operand arrays."""
     same = copy_rate(launch_bytes / 2, 200)
     return {"same_bytes_GBps": same, "hbm_resident_1GiB_GBps": copy_rate(1 << 30, 20),
+            "hbm_resident_30_streams_GBps": stream_rate(30, 160 ** 3 // 2, 10),
             "achieved_frac_of_same_bytes_copy": achieved / same}
 
 
@@ -1039,6 +1054,9 @@ def main():
     rec["roofline"]["hbm_resident"]["frac_of_1GiB_copy"] = (
         rec["roofline"]["hbm_resident"]["achieved"]
         / rec["roofline"]["copy_ceiling"]["hbm_resident_1GiB_GBps"])
+    rec["roofline"]["hbm_resident"]["frac_of_30_streams"] = (
+        rec["roofline"]["hbm_resident"]["achieved"]
+        / rec["roofline"]["copy_ceiling"]["hbm_resident_30_streams_GBps"])
     if not args.no_learned:
         rec["learned"] = learned_batch(args, log=_log)
     if not args.no_cpu:

@@ -109,6 +109,13 @@ int lceq_autotune_site_order(lceq_lattice *L, int n_or, int trials, double *ms);
  * bandwidth that the lattice launches are compared with.  Allocates and frees
  * two buffers of `bytes`. */
 int lceq_copy_bandwidth(int64_t bytes, int reps, double *us_per_copy);
+/* Measurement aid with no counterpart in the reference: `streams` (8, 16 or
+ * 30) device arrays of `elements` 16-byte values read at the same index by
+ * each thread, one 16-byte array written -- the access shape of a launch that
+ * gathers that many operand arrays -- 3 untimed passes, then `reps` timed
+ * with HIP events.  us_per_pass receives the average; (streams + 1) *
+ * elements * 16 bytes move per pass.  Allocates and frees its buffers. */
+int lceq_stream_bandwidth(int streams, int64_t elements, int reps, double *us_per_pass);
 
 /* ---- uniforms --------------------------------------------------------- */
 /* Counter-based Philox4x32-10 streams: draw k of site s in update pass P uses

@@ -26,6 +26,7 @@ SIGNATURES = {
     "lceq_get_site_order": [VP, IP],
     "lceq_autotune_site_order": [VP, I, I, DP],
     "lceq_copy_bandwidth": [I64, I, DP],
+    "lceq_stream_bandwidth": [I, I64, I, DP],
     "lceq_rng_philox": [VP, U64, U64],
     "lceq_rng_feed": [VP, DP, I64],
     "lceq_rng_state": [VP, C.POINTER(I64), C.POINTER(U64)],

@@ -226,6 +226,75 @@ extern "C" int lceq_copy_bandwidth(int64_t bytes, int reps, double* us_per_copy)
   return rc;
 }
 
+// S streams read at the same index (16 bytes each per thread), one 16-byte
+// write: the access shape of a launch that gathers S operand arrays
+constexpr int STREAM_MAX = 32;
+struct StreamPtrs {
+  const double2* p[STREAM_MAX];
+};
+template <int S>
+__global__ __launch_bounds__(256) void k_streams(StreamPtrs a, double2* __restrict__ out,
+                                                 int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double x = 0.0, y = 0.0;
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    double2 v = a.p[k][i];
+    x += v.x;
+    y += v.y;
+  }
+  out[i] = double2{x, y};
+}
+
+extern "C" int lceq_stream_bandwidth(int streams, int64_t elements, int reps,
+                                     double* us_per_pass) {
+  if (!us_per_pass) return fail(LCEQ_ERR_ARG, "null output pointer");
+  if (streams != 8 && streams != 16 && streams != 30)
+    return fail(LCEQ_ERR_ARG, "streams must be 8, 16 or 30");
+  if (elements < 1 || reps < 1 || (elements + 255) / 256 > INT32_MAX)
+    return fail(LCEQ_ERR_ARG, "elements must be positive and below 2^39, reps >= 1");
+  size_t bytes = (size_t)elements * 16;
+  int nb = (int)((elements + 255) / 256);
+  StreamPtrs a{};
+  double2* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = LCEQ_OK;
+  for (int k = 0; k < streams && rc == LCEQ_OK; ++k) {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess || hipMemset(q, 0, bytes) != hipSuccess)
+      rc = fail(LCEQ_ERR_HIP, "stream buffer %d could not be set up", k);
+    a.p[k] = static_cast<const double2*>(q);
+  }
+  if (rc == LCEQ_OK && (hipMalloc(&out, bytes) != hipSuccess ||
+                        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(LCEQ_ERR_HIP, "stream output or events could not be set up");
+  if (rc == LCEQ_OK) {
+    auto launch = [&]() {
+      switch (streams) {
+        case 8: hipLaunchKernelGGL(k_streams<8>, dim3(nb), dim3(256), 0, 0, a, out, elements); break;
+        case 16: hipLaunchKernelGGL(k_streams<16>, dim3(nb), dim3(256), 0, 0, a, out, elements); break;
+        default: hipLaunchKernelGGL(k_streams<30>, dim3(nb), dim3(256), 0, 0, a, out, elements); break;
+      }
+    };
+    for (int r = 0; r < 3; ++r) launch();
+    float el = 0.0f;
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&el, e0, e1) != hipSuccess)
+      rc = fail(LCEQ_ERR_HIP, "stream timing failed");
+    else
+      *us_per_pass = 1e3 * (double)el / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (int k = 0; k < streams; ++k)
+    if (a.p[k]) (void)hipFree(const_cast<double2*>(a.p[k]));
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
 // Time `trials` production sweeps (schedule 0, no measurement) at every
 // launch shape and keep the fastest; the fields, observables, pass counter
 // and history are restored, so tuning leaves the chain where it was.

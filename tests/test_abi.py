@@ -95,3 +95,11 @@ def test_coupling_maps():
     assert couplings_from_trace_convention(9.0, 0.35, -0.9, 0.1) == (9.0, 0.175, -0.45, 0.025)
     ba, b2, b4 = couplings_from_xy(9.0, 0.1, 0.5)
     assert ba == 9.0 and abs(b4 - 0.9) < 1e-15 and abs(b2 - 25.0856) < 1e-3
+
+
+def test_stream_bandwidth_arguments_checked_before_device_use(lib):
+    us = C.c_double(-1.0)
+    for streams, n, reps in ((5, 1 << 16, 5), (30, 0, 5), (16, 1 << 16, 0), (32, 1 << 16, 5)):
+        assert lib.lceq_stream_bandwidth(streams, n, reps, C.byref(us)) == -1
+    assert lib.lceq_stream_bandwidth(30, 1 << 16, 5, None) == -1
+    assert us.value == -1.0

@@ -106,3 +106,18 @@ def test_site_order_tuner_restores_the_chain():
         L.set_site_order(3)
     with pytest.raises(lceqcd_amd.LceqError, match="strip rows"):
         L.set_site_order(32)
+
+
+def test_stream_bandwidth_probe():
+    """The 30-stream read probe that the benchmark reports beside the copy
+    ceiling returns a finite time and a rate below the 8 TB/s peak for 30
+    streams of 2^20 16-byte values."""
+    import ctypes as C
+    from lceqcd_amd import _lib
+    us = C.c_double()
+    n = 1 << 20
+    print("stream probe", n, flush=True)
+    _lib.check(_lib.load().lceq_stream_bandwidth(30, n, 5, C.byref(us)))
+    rate = 31 * n * 16 / (us.value * 1e-6) / 1e9
+    print("stream probe GB/s", rate, flush=True)
+    assert np.isfinite(us.value) and 0.0 < rate < 20000.0
