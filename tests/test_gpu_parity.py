@@ -48,7 +48,10 @@ def _pair(make, omake, shape, seed, coup=COUP, asig=0.5):
     return G, O
 
 
-SHAPES = [(2, 2, 2), (2, 4, 6), (4, 4, 4), (6, 4, 8), (8, 8, 8), (16, 16, 16), (8, 8, 24)]
+# thin lattices (2 sites in two directions) exercise every periodic wrap of
+# the neighbour steps at once
+SHAPES = [(2, 2, 2), (2, 4, 6), (4, 4, 4), (6, 4, 8), (8, 8, 8), (16, 16, 16), (8, 8, 24),
+          (2, 2, 32), (32, 2, 2), (2, 32, 2)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=["x".join(map(str, s)) for s in SHAPES])
