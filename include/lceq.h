@@ -90,9 +90,10 @@ int lceq_get_shape(lceq_lattice *L, int *threads_per_block, int *xcd_order);
 int lceq_autotune(lceq_lattice *L, int n_or, int trials, double *ms);
 /* site order of the link and Higgs updates (no counterpart in the
  * reference): strip_rows = 0 takes the sites in index order; a power of two
- * >= 2 dividing ly takes the rows (y, z) of a parity in strips of that many y
- * rows, each strip advancing through z, so that the z neighbours of the sites
- * in flight stay in the L2 of a lattice larger than the Infinity Cache.
+ * >= 2 dividing ly (with ly * lz < 2^22) takes the rows (y, z) of a parity
+ * in strips of that many y rows, each strip advancing through z, so that the
+ * z neighbours of the sites in flight stay in the L2 of a lattice larger than
+ * the Infinity Cache.
  * Fields do not depend on the order; plaquette and Higgs sums are added in an
  * order-dependent way.  Default 0. */
 int lceq_set_site_order(lceq_lattice *L, int strip_rows);

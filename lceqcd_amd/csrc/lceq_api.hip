@@ -358,6 +358,12 @@ extern "C" int lceq_set_site_order(lceq_lattice* L, int strip_rows) {
   if (sy != 0 && (sy < 2 || (sy & (sy - 1)) || L->g.ly % sy))
     return fail(LCEQ_ERR_ARG, "strip rows must be 0 or a power of two >= 2 dividing ly = %d",
                 L->g.ly);
+  // order_site divides a launch position by lx/2 to get its row; the
+  // reciprocal division is exact for quotients below 2^22, so the rows
+  // ly * lz of a parity must stay below that
+  if (sy != 0 && (int64_t)L->g.ly * L->g.lz >= ((int64_t)1 << 22))
+    return fail(LCEQ_ERR_ARG, "strip rows need ly * lz < 2^22 (ly = %d, lz = %d)", L->g.ly,
+                L->g.lz);
   HIPCHK(hipStreamSynchronize(L->stream));
   L->strip = sy;
   return LCEQ_OK;
@@ -397,8 +403,8 @@ extern "C" int lceq_autotune_site_order(lceq_lattice* L, int n_or, int trials, d
   int best_sy = L->strip, rc = LCEQ_OK;
   for (int i = 0; i < 4 && rc == LCEQ_OK; ++i) {
     int sy = orders[i];
-    if (sy && L->g.ly % sy) {
-      if (ms) ms[i] = -1.0;
+    if (sy && (L->g.ly % sy || (int64_t)L->g.ly * L->g.lz >= ((int64_t)1 << 22))) {
+      if (ms) ms[i] = -1.0;  // the order does not fit (lceq_set_site_order's limits)
       continue;
     }
     L->strip = sy;

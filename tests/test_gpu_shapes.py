@@ -106,6 +106,13 @@ def test_site_order_tuner_restores_the_chain():
         L.set_site_order(3)
     with pytest.raises(lceqcd_amd.LceqError, match="strip rows"):
         L.set_site_order(32)
+    L.close()
+    # ly * lz = 2^22 rows per parity: the row division would no longer be exact
+    L = lceqcd_amd.Lattice(2, 2048, 2048)
+    with pytest.raises(lceqcd_amd.LceqError, match="ly \\* lz < 2\\^22"):
+        L.set_site_order(2)
+    assert L.get_site_order() == 0
+    L.close()
 
 
 def test_stream_bandwidth_probe():
