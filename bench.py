@@ -866,6 +866,20 @@ def copy_ceiling(launch_bytes, achieved):
             "achieved_frac_of_same_bytes_copy": achieved / same}
 
 
+def ceilings():
+    """The practical bandwidth ceilings the link launches are compared with:
+    plain device copies of 35.65 MB (one 64^3 OR launch's bytes) and 1 GiB,
+    and the multi-stream read probe with 8, 16 and 30 read streams of one
+    64^3 and one 160^3 parity class each (one write stream).  One JSON line,
+    GB/s."""
+    out = {"copy_35.65MB": copy_rate(35651584 / 2, 200), "copy_1GiB": copy_rate(1 << 30, 20)}
+    for L in (64, 160):
+        n = L ** 3 // 2
+        for k in (8, 16, 30):
+            out["streams%d_%dcubed_class" % (k, L)] = stream_rate(k, n, 200 if L == 64 else 10)
+    print(json.dumps({"workload": "ceilings", "GBps": out}), flush=True)
+
+
 def calibrate_copy():
     import torch
     n = 64 * 1024 * 1024  # 64 Mi doubles = 512 MiB per buffer
@@ -894,7 +908,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64", "scan",
-                             "mlpbatch", "plain", "gaugefix32", "production", "kernels"])
+                             "mlpbatch", "plain", "gaugefix32", "production", "kernels",
+                             "ceilings"])
     ap.add_argument("--sweeps", type=int, default=None,
                     help="sweeps per chain (learned workloads; default 60000 at 48^3, "
                          "20000 at 96^3: a few % statistical error on tau_int)")
@@ -940,6 +955,8 @@ def main():
         return scan(args)
     if args.workload == "kernels":
         return kernels(args)
+    if args.workload == "ceilings":
+        return ceilings()
     if args.workload == "mlpbatch":
         print(json.dumps({"workload": "mlpbatch", "learned": learned_batch(args, log=_log)}),
               flush=True)
