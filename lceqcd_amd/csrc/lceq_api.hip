@@ -910,12 +910,12 @@ extern "C" int lceq_multall(lceq_lattice* L, int kind, const double* op1, const 
     return fail(LCEQ_ERR_ARG, "bad multall arguments");
   if (n == 0) return LCEQ_OK;
   // operand extents: the largest site index referenced
-  int m1 = n, m2 = n;
+  int64_t m1 = n, m2 = n;
   for (int i = 0; i < n; ++i) {
     if (s1 && (s1[i] < 0)) return fail(LCEQ_ERR_ARG, "negative site index");
     if (s2 && (s2[i] < 0)) return fail(LCEQ_ERR_ARG, "negative site index");
-    if (s1 && s1[i] + 1 > m1) m1 = s1[i] + 1;
-    if (s2 && s2[i] + 1 > m2) m2 = s2[i] + 1;
+    if (s1 && (int64_t)s1[i] + 1 > m1) m1 = (int64_t)s1[i] + 1;
+    if (s2 && (int64_t)s2[i] + 1 > m2) m2 = (int64_t)s2[i] + 1;
   }
   int w1 = kind >= 4 ? 3 : 4;
   size_t bytes = ((size_t)w1 * m1 + 4 * (size_t)m2 + 4 * (size_t)n) * sizeof(double) +

@@ -329,6 +329,11 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
     memset(&R, 0, sizeof(R));
     R.g = geom_init(lx, ly, lz);
     R.nb = (R.g.n + TB_MIN - 1) / TB_MIN;
+    // one grid covers every replica's blocks (grid x below 2^31)
+    if ((int64_t)nb_total + R.nb > INT32_MAX) {
+      delete B;
+      return bfail(LCEQ_ERR_ARG, "replicas 0..%d need 2^31 or more blocks of %d sites", r, TB_MIN);
+    }
     R.blk0 = nb_total;
     R.pstride = R.nb;
     R.beta = 9.0;
