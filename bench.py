@@ -32,15 +32,19 @@ phase and 4 x 48^3 near the transition, 20000 sweeps each, one network):
 tau_int of Tr A^2, acceptance, independent samples/s and their ratio, also
 with the collection and training time charged (--no-learned leaves it out).
 
-    python bench.py --workload replicas96 [--steps K] [--hist-out FILE]
+    python bench.py --workload replicas96 [--steps K] [--group-mb M] [--hist-out FILE]
 
 measures the batched-replica scan instead: 96 independent lattices cycling
 through 16^3, 24^3, 32^3, 40^3, 48^3 and 24^2x96, on the grid
 beta_A in linspace(0.28, 0.40, 12) x beta_2 in linspace(-1.2, -0.6, 8)
 (trace convention, beta_G = 9, beta_4 = 0.1), seeds 12345 + r, all swept by
 the same launches (replayed hipGraphs of 8 sweeps, launch shape picked by
-the autotuner, which leaves the chains unchanged); value = aggregate
-site-updates/s.  --hist-out writes per-replica <Tr A^2> histograms (JSON).
+the autotuner, which leaves the chains unchanged), in replica groups of
+about M MiB of fields (the library's default without --group-mb; 0 = one
+group) so that a group's fields stay in the Infinity Cache through its
+sweep; value = aggregate site-updates/s.  The same batch is also timed as one
+group for comparison.  --hist-out writes per-replica <Tr A^2> histograms
+(JSON).
 
     python bench.py --workload mlp48 | mlp96 [--sweeps K]   (K: 60000 at 48^3, 20000 at 96^3)
 
@@ -206,6 +210,8 @@ def replicas96(args):
     B.set_couplings(coup)
     B.hotstart(seeds, 0.5)
     B.rng_philox(seeds)
+    if args.group_mb is not None:
+        B.set_group_bytes(int(args.group_mb * (1 << 20)))
     B.sweep(args.warmup, n_or=args.n_or, delta=0.4, measflag=0)
     if args.shape:
         t, x = (int(v) for v in args.shape.split(","))
@@ -221,6 +227,19 @@ def replicas96(args):
     ms = (t1 - t0) * 1e3 / args.steps
     V = B.total_sites
     hist = B.history(min(args.steps, 1024))
+    # the same batch swept as one group, for comparison (the chains go on)
+    group_bytes, first = B.groups()
+    one_ms = None
+    if len(first) > 1:
+        B.set_group_bytes(0)
+        n1 = min(args.steps, 48)
+        B.sweep(8, n_or=args.n_or, measflag=0)
+        B.sync()
+        t2 = time.perf_counter()
+        B.sweep(n1, n_or=args.n_or, measflag=0)
+        B.sync()
+        one_ms = (time.perf_counter() - t2) * 1e3 / n1
+        B.set_group_bytes(group_bytes)
     tra2 = hist[:, :, 2] / 2.0  # Tr A^2 = |a|^2 / 2 per site (A = a.sigma/2)
     rec = {
         "metric": METRIC,
@@ -237,8 +256,8 @@ def replicas96(args):
         "data": "synthetic",
         "config": {
             "workload": "96 replicas (16^3..48^3, 24^2x96) on a beta_A x beta_2 grid, 1 KP + %d OR "
-                        "+ ahiggs1 per sweep, hipGraph of 8 sweeps, autotuned launch shape"
-                        % args.n_or,
+                        "+ ahiggs1 per sweep, hipGraph of 8 sweeps, autotuned launch shape, "
+                        "replica groups of %d MiB of fields" % (args.n_or, group_bytes >> 20),
             "grid": {"beta_a": [float(x) for x in ba], "beta_2": [float(x) for x in b2],
                      "kind": args.grid},
             "total_sites": V,
@@ -246,6 +265,11 @@ def replicas96(args):
             "sweep_algorithmic_frac_of_hbm": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9
             / HBM_PEAK_GBPS,
             "tr_a2_mean_range": [float(tra2.mean(0).min()), float(tra2.mean(0).max())],
+            "replica_groups": {
+                "group_bytes": group_bytes, "groups": len(first), "first_replicas": first,
+                "field_MB": 120 * V / 1e6,
+                "one_group_ms_per_sweep": one_ms,
+                "speedup_over_one_group": None if one_ms is None else one_ms / ms},
             "launch_shape": {"threads_per_block": B.get_shape()[0], "xcd_order": B.get_shape()[1]},
             "autotune_ms_per_sweep": {"%d%s" % (t, "x" if x else ""): v
                                       for (t, x), v in tuned.items()},
@@ -918,6 +942,9 @@ def main():
     ap.add_argument("--therm", type=int, default=300,
                     help="thermalisation sweeps before training (learned workloads)")
     ap.add_argument("--hist-out", default=None)
+    ap.add_argument("--group-mb", type=float, default=None,
+                    help="replicas96: replica group size in MiB of fields (0: one group; "
+                         "default: the library's)")
     ap.add_argument("--scan-lattice", default="24,24,24")
     ap.add_argument("--order", type=int, default=0,
                     help="site order (strip rows, 0 = index order, -1 = the site-order "

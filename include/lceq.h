@@ -239,7 +239,8 @@ int lceq_profile_end(lceq_lattice *L, double *total_ms, int *launches);
 /* ---- batched replicas ---------------------------------------------------- */
 /* Many independent lattices of uneven sizes and couplings (a phase-diagram
  * scan, one Markov chain each) swept together: every (parity, mu) link
- * launch and every Higgs launch covers all replicas, and a sweep is one
+ * launch and every Higgs launch covers all replicas of a replica group (all
+ * replicas, when the batch fits the Infinity Cache), and eight sweeps are one
  * replayed hipGraph.  No reference counterpart (the reference runs one
  * lattice per MPI job).  Replica r follows exactly the chain of a single
  * lattice with the same seed under lceq_sweep(LCEQ_SWEEP_KP_OR_METRO) (and,
@@ -279,6 +280,18 @@ int lceq_batch_observables(lceq_batch *B, double *out);
  * order.  Default 256, XCD order. */
 int lceq_batch_set_shape(lceq_batch *B, int threads_per_block, int xcd_order);
 int lceq_batch_get_shape(lceq_batch *B, int *threads_per_block, int *xcd_order);
+/* replica groups (no counterpart in the reference): a sweep runs its
+ * launches over one run of consecutive replicas at a time, so that a batch
+ * larger than the 256 MB Infinity Cache is swept in groups whose fields stay
+ * in it from launch to launch.  The replicas are split into
+ * ceil(fields / group_bytes) runs of about equal field bytes (120 bytes per
+ * site); 0 sweeps the batch as one group.  The chains do not depend on the
+ * grouping.  Default 288 MiB. */
+int lceq_batch_set_group_bytes(lceq_batch *B, int64_t group_bytes);
+/* group_bytes, the number of groups, and the first replica of each of the
+ * first max_groups groups (any pointer may be NULL) */
+int lceq_batch_get_groups(lceq_batch *B, int64_t *group_bytes, int *ngroups, int *first_replica,
+                          int max_groups);
 /* autotuner: time `trials` sweeps of every shape on this batch, keep the
  * fastest; fields, observables and the sweep counter are restored.  ms (or
  * NULL) receives 8 times per sweep, shapes (64,128,256,512) x (launch order,

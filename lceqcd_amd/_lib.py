@@ -81,6 +81,8 @@ SIGNATURES = {
     "lceq_batch_observables": [VP, DP],
     "lceq_batch_set_shape": [VP, I, I],
     "lceq_batch_get_shape": [VP, IP, IP],
+    "lceq_batch_set_group_bytes": [VP, I64],
+    "lceq_batch_get_groups": [VP, C.POINTER(I64), IP, IP, I],
     "lceq_batch_autotune": [VP, I, I, DP],
     "lceq_batch_sweep_mlp": [VP, VP, I, I, I, I],
     "lceq_batch_mlp_collect": [VP, VP, I, I],

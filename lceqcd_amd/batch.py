@@ -125,6 +125,20 @@ class Batch:
         check(self._L.lceq_batch_get_shape(self.h, C.byref(t), C.byref(x)))
         return t.value, bool(x.value)
 
+    def set_group_bytes(self, group_bytes):
+        """Sweep the batch in runs of consecutive replicas of about
+        ``group_bytes`` of fields each (0: one group); the chains do not
+        depend on it."""
+        check(self._L.lceq_batch_set_group_bytes(self.h, int(group_bytes)))
+
+    def groups(self):
+        """(group_bytes, [first replica of each group])."""
+        gb, ng = C.c_int64(), C.c_int()
+        check(self._L.lceq_batch_get_groups(self.h, C.byref(gb), C.byref(ng), None, 0))
+        first = (C.c_int * max(1, ng.value))()
+        check(self._L.lceq_batch_get_groups(self.h, None, None, first, ng.value))
+        return gb.value, [int(first[g]) for g in range(ng.value)]
+
     def autotune(self, n_or=4, trials=3):
         """Time every launch shape on this batch and keep the fastest (the
         chains are left where they were).  Returns {(threads, xcd): ms/sweep}."""

@@ -1,10 +1,16 @@
 // lceq_batch.hip -- many independent lattices of uneven sizes and couplings
 // (a phase-diagram scan) swept together: one launch per (parity, mu) link
-// class and per Higgs parity covers every replica, so the device sees the sum
-// of all replicas' sites per launch.  Eight sweeps' launches are captured
-// once in a hipGraph and replayed; the Philox pass number comes from a
-// device-side pass counter (with the sweep counter that indexes the history
-// rows) so that the captured graph stays valid from sweep to sweep.  The launch shape (threads per block, XCD-aware block order) is a
+// class and per Higgs parity covers every replica of a replica group, so the
+// device sees the sum of the group's sites per launch.  A batch larger than
+// the 256 MB Infinity Cache is split into groups of about 288 MiB of fields,
+// and a sweep runs its launches group after group: the 33 launches of a
+// group's sweep then find its fields in the Infinity Cache instead of HBM
+// (the 539 MB, 96-replica scan sweeps 25% faster as two groups than as one),
+// at the cost of one more round of launches per group.  Eight sweeps'
+// launches are captured once in a hipGraph and replayed; the Philox pass
+// number comes from a device-side pass counter (with the sweep counter that
+// indexes the history rows) so that the captured graph stays valid from sweep
+// to sweep.  The launch shape (threads per block, XCD-aware block order) is a
 // runtime choice that lceq_batch_autotune picks by timing.
 //
 // Replica r with seed s_r follows exactly the trajectory that a single
@@ -15,6 +21,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -45,7 +52,8 @@ struct Replica {
 struct BatchCommon {
   const Replica* reps;
   const int* blk2rep;
-  int total_nb;
+  int total_nb;           // blocks of this launch (one replica group's)
+  int blk_base;           // global index of the launch's first block
   // device counters: ctr[0] sweeps (history row), ctr[1] the Philox pass
   // of the first launch of the current sweep; launch j of a sweep uses pass
   // ctr[1] + j, as the j-th next_pass() of a single lattice's sweep
@@ -54,7 +62,7 @@ struct BatchCommon {
 };
 
 __device__ __forceinline__ int batch_block(const BatchCommon& c, int& r, const Replica*& R) {
-  int gblk = c.xcd ? xcd_block(c.total_nb) : (int)blockIdx.x;
+  int gblk = c.blk_base + (c.xcd ? xcd_block(c.total_nb) : (int)blockIdx.x);
   r = c.blk2rep[gblk];
   R = c.reps + r;
   return gblk - R->blk0;
@@ -124,10 +132,10 @@ __global__ __launch_bounds__(TBX) void k_higgs_batch(BatchCommon c, int eo, int 
 // 256 per block) on the replica's fields, couplings, key and partial-sum
 // slots, one network for all replicas (the couplings are network inputs).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_higgs_mlp_batch(
-    BatchCommon c, const int* __restrict__ mblk2rep, int total_mnb,
+    BatchCommon c, const int* __restrict__ mblk2rep, int total_mnb, int mblk_base,
     const mlp::MlpPack* __restrict__ pack, int eo, int pass_in_sweep) {
   __shared__ mlp::MlpLds lds[4];
-  int gblk = c.xcd ? xcd_block(total_mnb) : (int)blockIdx.x;
+  int gblk = mblk_base + (c.xcd ? xcd_block(total_mnb) : (int)blockIdx.x);
   const Replica* R = c.reps + mblk2rep[gblk];
   mlp::MlpArgs a;
   a.g = R->g;
@@ -152,13 +160,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // 2: the learned proposal (Higgs observables always, as lceq_sweep_mlp
 // records them, no step adaptation; its partials come from R->mnb blocks).
 __global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restrict__ reps,
-                                                         int link_kind, int meas, int higgs_kind,
+                                                         int rep_base, int link_kind, int meas,
+                                                         int higgs_kind,
                                                          const uint64_t* __restrict__ ctr,
                                                          double* __restrict__ hist, int hist_cap,
                                                          int nrep) {
   constexpr int NV = 6 * 3 + 2 * 4;
   __shared__ double tot[32];
-  const Replica* R = reps + blockIdx.x;
+  const int r = rep_base + (int)blockIdx.x;
+  const Replica* R = reps + r;
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -223,7 +233,7 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const Replica* __restri
     obs[OBS_DELTA] = d;
   }
   if (meas == 1 && hist) {
-    double* row = hist + ((size_t)(ctr[0] % (uint64_t)hist_cap) * nrep + blockIdx.x) * LCEQ_OBS_N;
+    double* row = hist + ((size_t)(ctr[0] % (uint64_t)hist_cap) * nrep + r) * LCEQ_OBS_N;
     for (int i = 0; i < LCEQ_OBS_N; ++i) row[i] = obs[i];
   }
 }
@@ -251,6 +261,15 @@ __global__ void k_tick(uint64_t* ctr, int dsweep, int dpass) {
 struct lceq_batch;
 static int batch_upload_table(lceq_batch* B);
 
+// A run of consecutive replicas swept together: every launch of a sweep
+// covers one group, and the groups of a sweep run one after another.
+struct ReplicaGroup {
+  int rep0, nrep;  // replicas [rep0, rep0 + nrep)
+  int blk0, nb;    // their blocks of the update launches at the launch shape
+  int mblk0, mnb;  // and of the learned Higgs launches
+  int64_t bytes;   // their link and Higgs fields
+};
+
 struct lceq_batch {
   int nrep;
   std::vector<Replica> reps;  // host copy of the device table
@@ -272,7 +291,15 @@ struct lceq_batch {
   // learned Higgs launches: 256-site blocks, independent of the launch shape
   int* d_mblk2rep;
   int total_mnb;
+  // replica groups: fields of at most about group_bytes per group (0: one
+  // group), so that a group stays in the Infinity Cache through its sweep
+  int64_t group_bytes;
+  std::vector<ReplicaGroup> groups;
 };
+
+// Default group size: the fields of a group about fill the 256 MB Infinity
+// Cache (the 539 MB replica scan then runs as two groups of 270 MB).
+constexpr int64_t DEFAULT_GROUP_BYTES = (int64_t)288 << 20;
 
 constexpr int GRAPH_SWEEPS = 8;
 
@@ -281,6 +308,44 @@ static int batch_upload_table(lceq_batch* B) {
                       hipMemcpyHostToDevice, B->stream));
   BHIP(hipStreamSynchronize(B->stream));
   return LCEQ_OK;
+}
+
+// Split the replicas into ceil(total / group_bytes) runs of about equal
+// field bytes (one run when group_bytes is 0 or the batch fits).
+static void batch_groups(lceq_batch* B) {
+  std::vector<int64_t> by(B->nrep);
+  int64_t total = 0;
+  for (int r = 0; r < B->nrep; ++r) total += by[r] = 30 * 8 * (int64_t)B->reps[r].g.npad;
+  int64_t ng = B->group_bytes > 0 ? (total + B->group_bytes - 1) / B->group_bytes : 1;
+  ng = std::max<int64_t>(1, std::min<int64_t>(ng, B->nrep));
+  B->groups.clear();
+  int64_t cum = 0;
+  int r0 = 0;
+  for (int r = 0; r < B->nrep; ++r) {
+    cum += by[r];
+    int64_t g = (int64_t)B->groups.size();
+    int64_t left = B->nrep - 1 - r, todo = ng - 1 - g;
+    // close the group once it reaches its share of the total, or when the
+    // replicas left are just enough for one per group still to come
+    if (r == B->nrep - 1 ||
+        (g < ng - 1 && left >= todo && (cum * ng >= total * (g + 1) || left == todo))) {
+      ReplicaGroup G;
+      G.rep0 = r0;
+      G.nrep = r + 1 - r0;
+      const Replica& R0 = B->reps[r0];
+      G.blk0 = R0.blk0;
+      G.mblk0 = R0.mblk0;
+      G.nb = G.mnb = 0;
+      G.bytes = 0;
+      for (int k = r0; k <= r; ++k) {
+        G.nb += B->reps[k].nb;
+        G.mnb += B->reps[k].mnb;
+        G.bytes += by[k];
+      }
+      B->groups.push_back(G);
+      r0 = r + 1;
+    }
+  }
 }
 
 // Block tables for launch shape tb: replica r owns blocks [blk0, blk0 + nb)
@@ -301,6 +366,7 @@ static int batch_layout(lceq_batch* B, int tb, int xcd) {
   B->total_nb = nb_total;
   B->tb = tb;
   B->xcd = xcd;
+  batch_groups(B);
   if (B->graph) {
     BHIP(hipGraphExecDestroy(B->graph));
     B->graph = nullptr;
@@ -352,6 +418,7 @@ extern "C" int lceq_batch_create(int nrep, const int* dims, lceq_batch** out) {
   B->pool_doubles = pool;
   B->parts_doubles = parts;
   B->hist_cap = 1024;
+  B->group_bytes = DEFAULT_GROUP_BYTES;
   int rc = LCEQ_OK;
   auto alloc = [&](void** p, size_t bytes) {
     if (rc) return;
@@ -498,7 +565,7 @@ extern "C" int lceq_batch_get_higgs(lceq_batch* B, int r, double* a, int64_t ld)
 
 template <int TBX>
 static void enqueue_links(lceq_batch* B, const BatchCommon& c, int n_or, int measflag) {
-  dim3 grid(B->total_nb), blk(TBX);
+  dim3 grid(c.total_nb), blk(TBX);
   int pass = 0;
   for (int k = -1; k < n_or; ++k) {
     int kp = k < 0;
@@ -516,48 +583,57 @@ static void enqueue_links(lceq_batch* B, const BatchCommon& c, int n_or, int mea
   }
 }
 
-// ahiggs1 of parity eo for every replica at pass ctr[1] + pass_in_sweep
+// ahiggs1 of parity eo for the replicas of c at pass ctr[1] + pass_in_sweep
 template <int TBX>
 static void enqueue_higgs(lceq_batch* B, const BatchCommon& c, int eo, int pass_in_sweep,
                           int measflag) {
-  hipLaunchKernelGGL((k_higgs_batch<TBX>), dim3(B->total_nb), dim3(TBX), 0, B->stream, c, eo,
+  hipLaunchKernelGGL((k_higgs_batch<TBX>), dim3(c.total_nb), dim3(TBX), 0, B->stream, c, eo,
                      pass_in_sweep, measflag);
 }
 
+// launches over every replica
 static BatchCommon batch_common(const lceq_batch* B) {
-  return BatchCommon{B->d_reps, B->d_blk2rep, B->total_nb, B->d_ctr, B->xcd};
+  return BatchCommon{B->d_reps, B->d_blk2rep, B->total_nb, 0, B->d_ctr, B->xcd};
+}
+// launches over one replica group
+static BatchCommon group_common(const lceq_batch* B, const ReplicaGroup& G) {
+  return BatchCommon{B->d_reps, B->d_blk2rep, G.nb, G.blk0, B->d_ctr, B->xcd};
 }
 
-// Enqueue one sweep of every replica: 1 KP + n_or overrelaxation link
-// sweeps + the Higgs update (ahiggs1, or the learned proposal of `pack`),
-// then the per-replica finalize and the counter tick.  Passes as a single
-// lattice's sweep: 6 per link sweep, then one per Higgs parity.
+// Enqueue one sweep of every replica: for each replica group in turn, 1 KP
+// + n_or overrelaxation link sweeps + the Higgs update (ahiggs1, or the
+// learned proposal of `pack`) and the group's per-replica finalize; then the
+// counter tick.  Passes as a single lattice's sweep: 6 per link sweep, then
+// one per Higgs parity, the same for every group (the counters tick once).
 static void enqueue_sweep(lceq_batch* B, int n_or, int measflag,
                           const mlp::MlpPack* pack = nullptr) {
-  BatchCommon c = batch_common(B);
   int hp = 6 * (1 + n_or);  // pass of the first Higgs parity
-  switch (B->tb) {
-    case 64: enqueue_links<64>(B, c, n_or, measflag); break;
-    case 128: enqueue_links<128>(B, c, n_or, measflag); break;
-    case 512: enqueue_links<512>(B, c, n_or, measflag); break;
-    default: enqueue_links<256>(B, c, n_or, measflag); break;
-  }
-  for (int eo = 0; eo < 2; ++eo) {
-    if (pack) {
-      hipLaunchKernelGGL(k_higgs_mlp_batch, dim3(B->total_mnb), dim3(mlp::MLP_TB), 0, B->stream,
-                         c, B->d_mblk2rep, B->total_mnb, pack, eo, hp + eo);
-      continue;
-    }
-    switch (B->tb) {
-      case 64: enqueue_higgs<64>(B, c, eo, hp + eo, measflag); break;
-      case 128: enqueue_higgs<128>(B, c, eo, hp + eo, measflag); break;
-      case 512: enqueue_higgs<512>(B, c, eo, hp + eo, measflag); break;
-      default: enqueue_higgs<256>(B, c, eo, hp + eo, measflag); break;
-    }
-  }
   int link_kind = measflag ? (n_or ? 2 : 1) : 0;
-  hipLaunchKernelGGL(k_finalize_batch, dim3(B->nrep), dim3(FIN_TB), 0, B->stream, B->d_reps,
-                     link_kind, measflag, pack ? 2 : 1, B->d_ctr, B->hist, B->hist_cap, B->nrep);
+  for (const ReplicaGroup& G : B->groups) {
+    BatchCommon c = group_common(B, G);
+    switch (B->tb) {
+      case 64: enqueue_links<64>(B, c, n_or, measflag); break;
+      case 128: enqueue_links<128>(B, c, n_or, measflag); break;
+      case 512: enqueue_links<512>(B, c, n_or, measflag); break;
+      default: enqueue_links<256>(B, c, n_or, measflag); break;
+    }
+    for (int eo = 0; eo < 2; ++eo) {
+      if (pack) {
+        hipLaunchKernelGGL(k_higgs_mlp_batch, dim3(G.mnb), dim3(mlp::MLP_TB), 0, B->stream, c,
+                           B->d_mblk2rep, G.mnb, G.mblk0, pack, eo, hp + eo);
+        continue;
+      }
+      switch (B->tb) {
+        case 64: enqueue_higgs<64>(B, c, eo, hp + eo, measflag); break;
+        case 128: enqueue_higgs<128>(B, c, eo, hp + eo, measflag); break;
+        case 512: enqueue_higgs<512>(B, c, eo, hp + eo, measflag); break;
+        default: enqueue_higgs<256>(B, c, eo, hp + eo, measflag); break;
+      }
+    }
+    hipLaunchKernelGGL(k_finalize_batch, dim3(G.nrep), dim3(FIN_TB), 0, B->stream, B->d_reps,
+                       G.rep0, link_kind, measflag, pack ? 2 : 1, B->d_ctr, B->hist, B->hist_cap,
+                       B->nrep);
+  }
   hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, B->stream, B->d_ctr, 1, hp + 2);
 }
 
@@ -696,6 +772,25 @@ extern "C" int lceq_batch_get_shape(lceq_batch* B, int* threads_per_block, int* 
   if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
   if (threads_per_block) *threads_per_block = B->tb;
   if (xcd_order) *xcd_order = B->xcd;
+  return LCEQ_OK;
+}
+
+extern "C" int lceq_batch_set_group_bytes(lceq_batch* B, int64_t group_bytes) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  if (group_bytes < 0) return bfail(LCEQ_ERR_ARG, "group_bytes must be >= 0 (0: one group)");
+  BHIP(hipStreamSynchronize(B->stream));
+  B->group_bytes = group_bytes;
+  return batch_layout(B, B->tb, B->xcd);
+}
+
+extern "C" int lceq_batch_get_groups(lceq_batch* B, int64_t* group_bytes, int* ngroups,
+                                     int* first_replica, int max_groups) {
+  if (!B) return bfail(LCEQ_ERR_ARG, "null batch");
+  if (group_bytes) *group_bytes = B->group_bytes;
+  if (ngroups) *ngroups = (int)B->groups.size();
+  if (first_replica)
+    for (int g = 0; g < (int)B->groups.size() && g < max_groups; ++g)
+      first_replica[g] = B->groups[g].rep0;
   return LCEQ_OK;
 }
 
