@@ -88,6 +88,13 @@ Landau-fixed to 1e-12 with the correlators; 64^3 and 32^2x128 at beta_A =
 each routine K times on one lattice: the command the per-kernel rocprofv3
 counter passes profile (a few hundred dispatches).
 
+    python bench.py --workload ceilings
+    python bench.py --workload sizes [--steps K]
+
+the copy and multi-stream read ceilings; the production sweep on every
+single-lattice size of the benchmark configurations (16^3 to 96^3) and on
+128^3 and 160^3, one JSON line each.
+
     python bench.py --calibrate-copy
 
 copies a 512 MiB device buffer three times (a plain copy kernel of known
@@ -904,6 +911,51 @@ def ceilings():
     print(json.dumps({"workload": "ceilings", "GBps": out}), flush=True)
 
 
+SIZES = [(16, 16, 16), (24, 24, 96), (32, 32, 32), (48, 48, 48), (64, 64, 64), (32, 32, 128),
+         (96, 96, 96), (128, 128, 128), (160, 160, 160)]
+
+
+def sizes(args):
+    """The production sweep (1 KP + n_or OR + ahiggs1, hot start, the 64^3
+    line's couplings) on every single-lattice size of the benchmark
+    configurations, and on 128^3 and 160^3 (whose fields no cache holds):
+    autotuned launch shape, then args.steps timed sweeps (scaled down to
+    about 1.2e9 site-updates at the larger sizes) and the OR launch from HIP
+    events.  One JSON line."""
+    import lceqcd_amd
+    out = []
+    for shape in SIZES:
+        V = shape[0] * shape[1] * shape[2]
+        steps = max(10, min(args.steps, int(1.2e9 / (V * 1.0 + 1))))
+        L = lceqcd_amd.Lattice(*shape)
+        L.set_couplings(*couplings())
+        L.hotstart(12345, 0.5)
+        L.rng_philox(12345, 0)
+        d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, 10, n_or=args.n_or, delta=0.4, measflag=0)
+        L.autotune(n_or=args.n_or, trials=3 if V > 2e6 else 10)
+        L.sync()
+        t0 = time.perf_counter()
+        d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, steps, n_or=args.n_or, delta=d, measflag=0)
+        L.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        L.profile_begin(lceqcd_amd.KC_LINK_OR, 6 * args.n_or * steps)
+        d = L.sweep(lceqcd_amd.SWEEP_KP_OR_METRO, steps, n_or=args.n_or, delta=d, measflag=0)
+        or_ms, or_n = L.profile_end()
+        or_us = 1e3 * or_ms / max(or_n, 1)
+        rec = {"lattice": list(shape), "sites": V, "fields_MB": 120 * V / 1e6, "steps": steps,
+               "ms_per_sweep": ms, "site_updates_per_s": V / (ms * 1e-3),
+               "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
+               "or_launch_us": or_us,
+               "or_launch_GBps": LINK_PASS_BYTES_PER_SITE * V / (or_us * 1e-6) / 1e9,
+               "launch_shape": list(L.get_shape())}
+        _log("sizes %s: %.4f ms/sweep, %.3g site-updates/s, OR %.2f us"
+             % ("x".join(map(str, shape)), ms, rec["site_updates_per_s"], or_us))
+        out.append(rec)
+        L.close()
+    print(json.dumps({"workload": "sizes", "n_or": args.n_or, "peak_GBps": HBM_PEAK_GBPS,
+                      "sizes": out}), flush=True)
+
+
 def calibrate_copy():
     import torch
     n = 64 * 1024 * 1024  # 64 Mi doubles = 512 MiB per buffer
@@ -933,7 +985,7 @@ def main():
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64", "scan",
                              "mlpbatch", "plain", "gaugefix32", "production", "kernels",
-                             "ceilings"])
+                             "ceilings", "sizes"])
     ap.add_argument("--sweeps", type=int, default=None,
                     help="sweeps per chain (learned workloads; default 60000 at 48^3, "
                          "20000 at 96^3: a few % statistical error on tau_int)")
@@ -984,6 +1036,8 @@ def main():
         return kernels(args)
     if args.workload == "ceilings":
         return ceilings()
+    if args.workload == "sizes":
+        return sizes(args)
     if args.workload == "mlpbatch":
         print(json.dumps({"workload": "mlpbatch", "learned": learned_batch(args, log=_log)}),
               flush=True)
