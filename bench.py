@@ -31,6 +31,9 @@ plain sweep on the same replicas (4 x 48^3 at beta_A = 0.34 in the broken
 phase and 4 x 48^3 near the transition, 20000 sweeps each, one network):
 tau_int of Tr A^2, acceptance, independent samples/s and their ratio, also
 with the collection and training time charged (--no-learned leaves it out).
+The replicas96 object times the 96-replica scan (539 MB of fields) in the
+library's default replica groups and as one group, 48 sweeps each
+(--no-replicas leaves it out).
 
     python bench.py --workload replicas96 [--steps K] [--group-mb M] [--hist-out FILE]
 
@@ -291,6 +294,43 @@ def replicas96(args):
                  "mean": float(tra2[:, r].mean()), "std": float(tra2[:, r].std())}
                 for r in range(nrep)]}, f)
     print(json.dumps(rec), flush=True)
+
+
+def replica_groups_summary(n_or, steps=48):
+    """The 96-replica scan (literal grid) swept in the library's default
+    replica groups and as one group, for the default line: autotuned launch
+    shape, `steps` timed sweeps each way (8-sweep graphs)."""
+    import lceqcd_amd
+    from lceqcd_amd.workloads import c4_replicas
+    shapes, coup, seeds, _ = c4_replicas(96)
+    B = lceqcd_amd.Batch(shapes)
+    B.set_couplings(coup)
+    B.hotstart(seeds, 0.5)
+    B.rng_philox(seeds)
+    B.sweep(8, n_or=n_or, delta=0.4, measflag=0)
+    B.autotune(n_or=n_or, trials=2)
+
+    def timed():
+        B.sweep(8, n_or=n_or, measflag=0)
+        B.sync()
+        t0 = time.perf_counter()
+        B.sweep(steps, n_or=n_or, measflag=0)
+        B.sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms = timed()
+    group_bytes, first = B.groups()
+    B.set_group_bytes(0)
+    one_ms = timed()
+    V = B.total_sites
+    B.close()
+    return {"workload": "96 replicas (16^3..48^3, 24^2x96), 1 KP + %d OR + ahiggs1, "
+                        "hipGraph of 8 sweeps, autotuned launch shape" % n_or,
+            "total_sites": V, "field_MB": 120 * V / 1e6, "steps": steps,
+            "group_bytes": group_bytes, "groups": len(first),
+            "ms_per_sweep": ms, "site_updates_per_s": V / (ms * 1e-3),
+            "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
+            "one_group_ms_per_sweep": one_ms, "speedup_over_one_group": one_ms / ms}
 
 
 def _batch_history(B, nsweep, step_fn, chunk=1024):
@@ -1008,6 +1048,8 @@ def main():
                          "throughout) or one that straddles the transition")
     ap.add_argument("--no-learned", action="store_true",
                     help="leave the batched learned-vs-plain object out of the default line")
+    ap.add_argument("--no-replicas", action="store_true",
+                    help="leave the 96-replica object out of the default line")
     ap.add_argument("--learned-lattice", type=int, default=48)
     ap.add_argument("--learned-replicas", type=int, default=4,
                     help="replicas per coupling of the batched learned comparison")
@@ -1157,6 +1199,11 @@ def main():
         / rec["roofline"]["copy_ceiling"]["hbm_resident_30_streams_GBps"])
     if not args.no_learned:
         rec["learned"] = learned_batch(args, log=_log)
+    if not args.no_replicas:
+        rec["replicas96"] = replica_groups_summary(args.n_or)
+        _log("replicas96: %.4f ms/sweep in %d groups, %.4f ms as one group"
+             % (rec["replicas96"]["ms_per_sweep"], rec["replicas96"]["groups"],
+                rec["replicas96"]["one_group_ms_per_sweep"]))
     if not args.no_cpu:
         one = cpu_baseline(shape, args.cpu_seconds, args.n_or)
         many = cpu_baseline_slabs(shape, args.cpu_seconds, args.n_or, args.cpu_procs)
