@@ -93,10 +93,12 @@ counter passes profile (a few hundred dispatches).
 
     python bench.py --workload ceilings
     python bench.py --workload sizes [--steps K]
+    python bench.py --workload groups
 
 the copy and multi-stream read ceilings; the production sweep on every
 single-lattice size of the benchmark configurations (16^3 to 96^3) and on
-128^3 and 160^3, one JSON line each.
+128^3 and 160^3; replica batches of 378-755 MB swept in the default replica
+groups and as one group; one JSON line each.
 
     python bench.py --calibrate-copy
 
@@ -331,6 +333,54 @@ def replica_groups_summary(n_or, steps=48):
             "ms_per_sweep": ms, "site_updates_per_s": V / (ms * 1e-3),
             "sweep_algorithmic_GBps": SWEEP_BYTES_PER_SITE * V / (ms * 1e-3) / 1e9,
             "one_group_ms_per_sweep": one_ms, "speedup_over_one_group": one_ms / ms}
+
+
+GROUP_BATCHES = [("4x96^3", [(96, 96, 96)] * 4), ("2x128^3", [(128, 128, 128)] * 2),
+                 ("3x128^3", [(128, 128, 128)] * 3), ("12x64^3", [(64, 64, 64)] * 12),
+                 ("96x32^3", [(32, 32, 32)] * 96)]
+
+
+def groups(args):
+    """Replica batches of several compositions larger than the Infinity
+    Cache, swept in the library's default replica groups and as one group
+    (256 threads, XCD order, 1 KP + n_or OR + ahiggs1, best of three runs of
+    24 sweeps each way).  One JSON line."""
+    import lceqcd_amd
+    from lceqcd_amd.workloads import coupling
+
+    def timed(B, nsw=24):
+        best = 1e9
+        for _ in range(3):
+            B.sweep(8, n_or=args.n_or, measflag=0)
+            B.sync()
+            t0 = time.perf_counter()
+            B.sweep(nsw, n_or=args.n_or, measflag=0)
+            B.sync()
+            best = min(best, (time.perf_counter() - t0) * 1e3 / nsw)
+        return best
+
+    out = {}
+    for name, shapes in GROUP_BATCHES:
+        n = len(shapes)
+        B = lceqcd_amd.Batch(shapes)
+        B.set_couplings([coupling(0.34, -0.9 + 0.02 * r) for r in range(n)])
+        B.hotstart([5 + r for r in range(n)], 0.5)
+        B.rng_philox([5 + r for r in range(n)])
+        B.set_shape(256, True)
+        B.sweep(8, n_or=args.n_or, delta=0.4, measflag=0)
+        g_ms = timed(B)
+        group_bytes, first = B.groups()
+        B.set_group_bytes(0)
+        one_ms = timed(B)
+        V = sum(x * y * z for x, y, z in shapes)
+        out[name] = {"field_MB": 120 * V / 1e6, "group_bytes": group_bytes,
+                     "first_replicas": first, "grouped_ms_per_sweep": g_ms,
+                     "one_group_ms_per_sweep": one_ms, "speedup": one_ms / g_ms,
+                     "grouped_site_updates_per_s": V / (g_ms * 1e-3)}
+        _log("groups %s: %d groups %.4f ms, one group %.4f ms"
+             % (name, len(first), g_ms, one_ms))
+        B.close()
+    print(json.dumps({"workload": "groups", "batches": out}), flush=True)
 
 
 def _batch_history(B, nsweep, step_fn, chunk=1024):
@@ -1025,7 +1075,7 @@ def main():
     ap.add_argument("--workload", default="single64",
                     choices=["single64", "replicas96", "mlp48", "mlp96", "refsched64", "scan",
                              "mlpbatch", "plain", "gaugefix32", "production", "kernels",
-                             "ceilings", "sizes"])
+                             "ceilings", "sizes", "groups"])
     ap.add_argument("--sweeps", type=int, default=None,
                     help="sweeps per chain (learned workloads; default 60000 at 48^3, "
                          "20000 at 96^3: a few % statistical error on tau_int)")
@@ -1080,6 +1130,8 @@ def main():
         return ceilings()
     if args.workload == "sizes":
         return sizes(args)
+    if args.workload == "groups":
+        return groups(args)
     if args.workload == "mlpbatch":
         print(json.dumps({"workload": "mlpbatch", "learned": learned_batch(args, log=_log)}),
               flush=True)
