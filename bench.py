@@ -340,13 +340,17 @@ GROUP_BATCHES = [("4x96^3", [(96, 96, 96)] * 4), ("2x128^3", [(128, 128, 128)] *
                  ("96x32^3", [(32, 32, 32)] * 96)]
 
 
+GROUP_SIZES_MIB = [0, 128, 192, 256, 288, 384]
+
+
 def groups(args):
-    """Replica batches of several compositions larger than the Infinity
-    Cache, swept in the library's default replica groups and as one group
-    (256 threads, XCD order, 1 KP + n_or OR + ahiggs1, best of three runs of
-    24 sweeps each way).  One JSON line."""
+    """Replica groups (256 threads, XCD order, 1 KP + n_or OR + ahiggs1, best
+    of three runs of 24 sweeps per point): the 96-replica scan and 48 x 48^3
+    at group sizes of 0 (one group) to 384 MiB of fields, then batches of
+    several compositions larger than the Infinity Cache in the library's
+    default groups and as one group.  One JSON line."""
     import lceqcd_amd
-    from lceqcd_amd.workloads import coupling
+    from lceqcd_amd.workloads import c4_replicas, coupling
 
     def timed(B, nsw=24):
         best = 1e9
@@ -359,6 +363,27 @@ def groups(args):
             best = min(best, (time.perf_counter() - t0) * 1e3 / nsw)
         return best
 
+    sweep = {}
+    c4_shapes, c4_coup, c4_seeds, _ = c4_replicas(96)
+    big48 = [(48, 48, 48)] * 48
+    for name, shapes, coup, seeds in (
+            ("96 replicas", c4_shapes, c4_coup, c4_seeds),
+            ("48x48^3", big48, [coupling(0.34, -0.9 + 0.02 * r) for r in range(48)],
+             [9 + r for r in range(48)])):
+        B = lceqcd_amd.Batch(shapes)
+        B.set_couplings(coup)
+        B.hotstart(seeds, 0.5)
+        B.rng_philox(seeds)
+        B.set_shape(256, True)
+        B.sweep(8, n_or=args.n_or, delta=0.4, measflag=0)
+        pts = {}
+        for mib in GROUP_SIZES_MIB:
+            B.set_group_bytes(mib << 20)
+            ms = timed(B)
+            pts[mib] = {"groups": len(B.groups()[1]), "ms_per_sweep": ms}
+            _log("groups %s at %d MiB: %d groups %.4f ms" % (name, mib, pts[mib]["groups"], ms))
+        sweep[name] = pts
+        B.close()
     out = {}
     for name, shapes in GROUP_BATCHES:
         n = len(shapes)
@@ -380,7 +405,7 @@ def groups(args):
         _log("groups %s: %d groups %.4f ms, one group %.4f ms"
              % (name, len(first), g_ms, one_ms))
         B.close()
-    print(json.dumps({"workload": "groups", "batches": out}), flush=True)
+    print(json.dumps({"workload": "groups", "group_sizes": sweep, "batches": out}), flush=True)
 
 
 def _batch_history(B, nsweep, step_fn, chunk=1024):
