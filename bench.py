@@ -33,7 +33,10 @@ tau_int of Tr A^2, acceptance, independent samples/s and their ratio, also
 with the collection and training time charged (--no-learned leaves it out).
 The replicas96 object times the 96-replica scan (539 MB of fields) in the
 library's default replica groups and as one group, 48 sweeps each
-(--no-replicas leaves it out).
+(--no-replicas leaves it out).  The reference_schedule object times the
+reference's own schedule (ahiggs2, ahiggs3, agauge, renorm) at 64^3 on the
+GPU and the reference's Fortran sweep of it under mpiexec on --cpu-procs host
+cores with its halo exchanges (--no-cpu leaves it out, with cpu_baseline).
 
     python bench.py --workload replicas96 [--steps K] [--group-mb M] [--hist-out FILE]
 
@@ -581,6 +584,43 @@ def cpu_baseline_reference_mpi(shape, procs_list, seconds):
                               "halo exchanges, %d timed sweeps, slowest rank"
                               % ((procs,) + tuple(shape) + tuple(grid) + (k,))})
     return out
+
+
+def reference_schedule_summary(seconds, procs=16):
+    """For the default line: the reference schedule (f/main.f:37-41:
+    ahiggs2, ahiggs3, agauge, renorm) at 64^3 on the GPU, and the
+    reference's own Fortran sweep of the same schedule under MPI on `procs`
+    host cores with its halo exchanges (about `seconds` of sweeps), at the
+    couplings refmpi.f sets."""
+    import lceqcd_amd
+    shape = (64, 64, 64)
+    V = shape[0] * shape[1] * shape[2]
+    L = lceqcd_amd.Lattice(*shape)
+    L.set_couplings(9.0, 0.175, -0.45, 0.025)
+    L.hotstart(314159, 0.5)
+    L.rng_philox(314159, 0)
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, 20, measflag=1)
+    L.history(20)
+    L.sync()
+    n = 200
+    t0 = time.perf_counter()
+    L.sweep(lceqcd_amd.SWEEP_REFERENCE, n, measflag=1)
+    L.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    L.history(n)
+    L.close()
+    rec = {"workload": "64^3 reference schedule (ahiggs2, ahiggs3, agauge, renorm), hot start, "
+                       "beta=9 beta_a=0.175 beta_2=-0.45 beta_4=0.025, %d sweeps" % n,
+           "gpu_ms_per_sweep": ms, "gpu_site_updates_per_s": V / (ms * 1e-3)}
+    try:
+        mpi = cpu_baseline_reference_mpi(shape, [procs], seconds)
+    except Exception as e:  # noqa: BLE001 - recorded, the line still prints
+        rec["reference_mpi_error"] = str(e)[-300:]
+        return rec
+    if mpi:
+        rec["reference_mpi"] = mpi[-1]
+        rec["gpu_over_reference_mpi"] = rec["gpu_site_updates_per_s"] / mpi[-1]["value"]
+    return rec
 
 
 def refsched64(args):
@@ -1282,6 +1322,10 @@ def main():
              % (rec["replicas96"]["ms_per_sweep"], rec["replicas96"]["groups"],
                 rec["replicas96"]["one_group_ms_per_sweep"]))
     if not args.no_cpu:
+        rec["reference_schedule"] = reference_schedule_summary(args.cpu_seconds, args.cpu_procs)
+        _log("reference schedule: GPU %.4f ms/sweep, reference under MPI %s"
+             % (rec["reference_schedule"]["gpu_ms_per_sweep"],
+                rec["reference_schedule"].get("reference_mpi", {}).get("seconds_per_sweep")))
         one = cpu_baseline(shape, args.cpu_seconds, args.n_or)
         many = cpu_baseline_slabs(shape, args.cpu_seconds, args.n_or, args.cpu_procs)
         rec["cpu_baseline"] = many if many is not None else one
