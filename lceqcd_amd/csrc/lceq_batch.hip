@@ -316,7 +316,8 @@ static void batch_groups(lceq_batch* B) {
   std::vector<int64_t> by(B->nrep);
   int64_t total = 0;
   for (int r = 0; r < B->nrep; ++r) total += by[r] = 30 * 8 * (int64_t)B->reps[r].g.npad;
-  int64_t ng = B->group_bytes > 0 ? (total + B->group_bytes - 1) / B->group_bytes : 1;
+  const int64_t gb = B->group_bytes;
+  int64_t ng = gb > 0 ? total / gb + (total % gb != 0) : 1;
   ng = std::max<int64_t>(1, std::min<int64_t>(ng, B->nrep));
   B->groups.clear();
   int64_t cum = 0;
