@@ -5,6 +5,17 @@
 // stream; here it derives seeds with the reference's recipe.  Arithmetic is
 // single precision as in the reference build: every state value is a
 // multiple of 2^-24 in [0,1), so the subtractions are exact.
+//
+// The algorithm is the published one: G. Marsaglia and A. Zaman, Ann. Appl.
+// Probab. 1 (1991) 462 (subtract-with-borrow, lags 24 and 10, base 2^24);
+// M. Luscher, Comput. Phys. Commun. 79 (1994) 100 (luxury levels: of every
+// 24 + skip numbers only 24 are delivered, p = 24, 48, 97, 223, 389); F. James,
+// Comput. Phys. Commun. 79 (1994) 111 (the portable implementation f/luxury.f
+// follows: seeding by a multiplicative congruential generator, the 25-word
+// restart vector, the initialisation by (level, seed, k1, k2)).  Its output
+// bits and restart vectors are an interface (seeds in the reference's files),
+// so the lags, carry, skip counts and counters keep that implementation's
+// names and steps.
 #include <stdint.h>
 #include <string.h>
 
