@@ -536,13 +536,14 @@ def cpu_baseline_reference_slabs(L, seconds, procs):
 MPI_GRIDS = {1: (1, 1, 1), 2: (1, 1, 2), 4: (1, 2, 2), 8: (2, 2, 2), 16: (2, 2, 4)}
 
 
-def cpu_baseline_reference_mpi(shape, procs_list, seconds):
+def cpu_baseline_reference_mpi(shape, procs_list, seconds, timeout=600):
     """The reference's own sweep under MPI with its halo exchanges
     (oracle/_ref/refmpi32_64: its ahiggs2, ahiggs3, agauge, renorm and
     typeconstruct, adresses as shipped, 4-byte reals, the rank grid of
     f/topology.f), run with mpiexec on 1..16 host cores; about `seconds` of
-    sweeps per rank count.  Returns one record per rank count, or None when
-    the build or mpiexec is absent."""
+    sweeps per rank count, each mpiexec run limited to `timeout` seconds.
+    Returns one record per rank count, or None when the build or mpiexec is
+    absent."""
     import resource
     import subprocess
     exe = os.path.join(ROOT, "oracle", "_ref", "refmpi32_64")
@@ -559,7 +560,7 @@ def cpu_baseline_reference_mpi(shape, procs_list, seconds):
         cmd = [mpiexec, "-n", str(procs), exe] + [str(v) for v in shape + grid] + [
             str(nwarm), str(nsweep)]
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                           timeout=600, preexec_fn=stack, cwd=ROOT)
+                           timeout=timeout, preexec_fn=stack, cwd=ROOT)
         line = [x for x in r.stdout.splitlines() if x.startswith("refmpi")]
         if r.returncode != 0 or not line:
             raise RuntimeError("refmpi failed: %s %s" % (r.stdout[-500:], r.stderr[-500:]))
@@ -613,7 +614,7 @@ def reference_schedule_summary(seconds, procs=16):
                        "beta=9 beta_a=0.175 beta_2=-0.45 beta_4=0.025, %d sweeps" % n,
            "gpu_ms_per_sweep": ms, "gpu_site_updates_per_s": V / (ms * 1e-3)}
     try:
-        mpi = cpu_baseline_reference_mpi(shape, [procs], seconds)
+        mpi = cpu_baseline_reference_mpi(shape, [procs], seconds, timeout=max(60.0, 4 * seconds))
     except Exception as e:  # noqa: BLE001 - recorded, the line still prints
         rec["reference_mpi_error"] = str(e)[-300:]
         return rec
