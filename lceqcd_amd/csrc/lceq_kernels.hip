@@ -38,7 +38,15 @@ struct LinkArgs {
   double* part;  // partial sums of this launch's slot: [LCEQ_NQ][pstride]
   int pstride, nb, xcd;
   SiteOrder order;
+  int early;  // the lattice's fields fit the Infinity Cache (link_update_site)
 };
+
+// The lattice's link and Higgs fields fit the 256 MB Infinity Cache: its
+// launches are latency-bound, and the early issue of the operands at the
+// site itself pays (link_update_site, higgs_update_site)
+static bool cache_resident(const lceq_lattice* L) {
+  return 30.0 * 8.0 * (double)L->g.npad <= 256.0 * 1024.0 * 1024.0;
+}
 
 // Block index in the launch's block order: XCD-aware (contiguous sites per
 // XCD) or the launch order
@@ -48,7 +56,7 @@ __device__ __forceinline__ int shape_block(int nb, int xcd) {
 }
 
 // One site per thread, TBX sites per block
-template <bool KP, bool FED, int TBX, int MU>
+template <bool KP, bool FED, int TBX, int MU, bool EARLY>
 __global__ __launch_bounds__(TBX) void k_link(LinkArgs a) {
   int blk = shape_block(a.nb, a.xcd);
   unsigned P = (unsigned)(blk * TBX + threadIdx.x);
@@ -56,22 +64,31 @@ __global__ __launch_bounds__(TBX) void k_link(LinkArgs a) {
   if (P < (unsigned)a.site.g.n) {
     LinkSite p = a.site;
     p.mu = MU;
-    link_update_site<KP, FED>(p, order_site(p.g, a.order, P), red);
+    link_update_site<KP, FED, EARLY>(p, order_site(p.g, a.order, P), red);
   }
   if (a.reduce) block_sum_store<3, TBX / 64>(red, a.part, a.pstride, blk);
 }
 
 int shape_blocks(const lceq_lattice* L) { return (L->g.n + L->tb - 1) / L->tb; }
 
-template <bool KP, bool FED, int MU>
-static void link_dispatch_mu(lceq_lattice* L, const LinkArgs& a) {
+template <bool KP, bool FED, int MU, bool EARLY>
+static void link_dispatch_tb(lceq_lattice* L, const LinkArgs& a) {
   dim3 grid(a.nb);
   switch (L->tb) {
-    case 64: hipLaunchKernelGGL((k_link<KP, FED, 64, MU>), grid, dim3(64), 0, L->stream, a); break;
-    case 128: hipLaunchKernelGGL((k_link<KP, FED, 128, MU>), grid, dim3(128), 0, L->stream, a); break;
-    case 512: hipLaunchKernelGGL((k_link<KP, FED, 512, MU>), grid, dim3(512), 0, L->stream, a); break;
-    default: hipLaunchKernelGGL((k_link<KP, FED, 256, MU>), grid, dim3(256), 0, L->stream, a); break;
+    case 64: hipLaunchKernelGGL((k_link<KP, FED, 64, MU, EARLY>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_link<KP, FED, 128, MU, EARLY>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_link<KP, FED, 512, MU, EARLY>), grid, dim3(512), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((k_link<KP, FED, 256, MU, EARLY>), grid, dim3(256), 0, L->stream, a); break;
   }
+}
+
+// the early issue exists for overrelaxation only (EARLY = !KP there)
+template <bool KP, bool FED, int MU>
+static void link_dispatch_mu(lceq_lattice* L, const LinkArgs& a) {
+  if (a.early)
+    link_dispatch_tb<KP, FED, MU, !KP>(L, a);
+  else
+    link_dispatch_tb<KP, FED, MU, false>(L, a);
 }
 
 // The direction is a template argument: the neighbour steps and link offsets
@@ -106,6 +123,7 @@ void launch_link(lceq_lattice* L, int kp, int eo, int mu, int measflag, int reno
   a.nb = shape_blocks(L);
   a.xcd = L->xcd;
   a.order = site_order(L->g, L->strip);
+  a.early = cache_resident(L) ? 1 : 0;
   int kc = kp ? KC_LINK_KP : KC_LINK_OR;
   prof_begin(L, kc);
   prof_count(L, kc);
@@ -282,27 +300,37 @@ struct HiggsArgs {
   double* part;
   int pstride, nb, xcd;
   SiteOrder order;
+  int early;  // as LinkArgs::early
 };
 
-template <int KIND, bool FED, int TBX>
+template <int KIND, bool FED, int TBX, bool EARLY>
 __global__ __launch_bounds__(TBX) void k_higgs(HiggsArgs a) {
   int blk = shape_block(a.nb, a.xcd);
   unsigned P = (unsigned)(blk * TBX + threadIdx.x);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
   if (P < (unsigned)a.site.g.n)
-    higgs_update_site<KIND, FED>(a.site, order_site(a.site.g, a.order, P), red);
+    higgs_update_site<KIND, FED, EARLY>(a.site, order_site(a.site.g, a.order, P), red);
   block_sum_store<4, TBX / 64>(red, a.part, a.pstride, blk);
 }
 
-template <int KIND, bool FED>
-static void higgs_dispatch(lceq_lattice* L, const HiggsArgs& a) {
+template <int KIND, bool FED, bool EARLY>
+static void higgs_dispatch_tb(lceq_lattice* L, const HiggsArgs& a) {
   dim3 grid(a.nb);
   switch (L->tb) {
-    case 64: hipLaunchKernelGGL((k_higgs<KIND, FED, 64>), grid, dim3(64), 0, L->stream, a); break;
-    case 128: hipLaunchKernelGGL((k_higgs<KIND, FED, 128>), grid, dim3(128), 0, L->stream, a); break;
-    case 512: hipLaunchKernelGGL((k_higgs<KIND, FED, 512>), grid, dim3(512), 0, L->stream, a); break;
-    default: hipLaunchKernelGGL((k_higgs<KIND, FED, 256>), grid, dim3(256), 0, L->stream, a); break;
+    case 64: hipLaunchKernelGGL((k_higgs<KIND, FED, 64, EARLY>), grid, dim3(64), 0, L->stream, a); break;
+    case 128: hipLaunchKernelGGL((k_higgs<KIND, FED, 128, EARLY>), grid, dim3(128), 0, L->stream, a); break;
+    case 512: hipLaunchKernelGGL((k_higgs<KIND, FED, 512, EARLY>), grid, dim3(512), 0, L->stream, a); break;
+    default: hipLaunchKernelGGL((k_higgs<KIND, FED, 256, EARLY>), grid, dim3(256), 0, L->stream, a); break;
   }
+}
+
+// the early issue exists for ahiggs1 only (EARLY = KIND == 1 there)
+template <int KIND, bool FED>
+static void higgs_dispatch(lceq_lattice* L, const HiggsArgs& a) {
+  if (a.early)
+    higgs_dispatch_tb<KIND, FED, KIND == 1>(L, a);
+  else
+    higgs_dispatch_tb<KIND, FED, false>(L, a);
 }
 
 void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double* fed,
@@ -326,6 +354,7 @@ void launch_higgs(lceq_lattice* L, int kind, int eo, int measflag, const double*
   a.nb = shape_blocks(L);
   a.xcd = L->xcd;
   a.order = site_order(L->g, L->strip);
+  a.early = cache_resident(L) ? 1 : 0;
   prof_begin(L, KC_HIGGS);
   prof_count(L, KC_HIGGS);
   if (kind == 1) {
