@@ -248,22 +248,28 @@ __device__ __forceinline__ void link_finish(const LinkSite& p, unsigned s, const
 }
 
 // One site: all operand loads are issued before any arithmetic, so that one
-// memory latency covers the site; the uniform phase runs under it.  EARLY
-// (overrelaxation only): the link and the Higgs triple at the site itself
-// go out ahead of the neighbour index arithmetic.  That pays where the
-// fields sit in the Infinity Cache (the OR launch 2.4% faster at 64^3, 2.2%
-// at 128^3) and costs 1.4% at 160^3, where HBM bounds the launch; the
-// heatbath was slower with it at every size measured.
+// memory latency covers the site; the uniform phase runs under it.
+// EARLY selects the schedule for launches of many waves whose fields sit in
+// the Infinity Cache (the launchers pick it per lattice):
+// - overrelaxation issues the link and the Higgs triple at the site itself
+//   ahead of the neighbour index arithmetic (the OR launch 2.4% faster at
+//   64^3, 2.2% at 128^3; 1.4% slower at 160^3, where HBM bounds it; the
+//   heatbath was slower with it);
+// - a fence keeps the uniform phase ahead of the update arithmetic: without
+//   it the compiler sinks part of the Philox rounds below the last operand
+//   wait, into the launch's exposed tail (KP 7.13 -> 6.91 us, OR 5.13 ->
+//   5.00 us at 64^3).
+// Launches of fewer than 512 waves (below 40^3), where one wave's chain is
+// the launch, keep the previous schedule: alternated A/B runs there were
+// within their 10% run-to-run spread.  The fields are the same bits either
+// way.
 template <bool KP, bool FED, bool EARLY = true>
 __device__ __forceinline__ void link_update_site(const LinkSite& p, unsigned s, double (&red)[3]) {
   LinkOperands o;
   link_load<KP, FED, EARLY>(p, s, o);
   __builtin_amdgcn_sched_barrier(0);
   link_prepare<KP, FED>(p, s, o);
-  // Without this fence the compiler sinks part of the Philox rounds below
-  // the last operand wait, into the launch's exposed tail: KP 7.13 -> 6.91
-  // us, OR 5.13 -> 5.00 us at 64^3 with it (same bits).
-  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (EARLY) __builtin_amdgcn_sched_barrier(0);
   link_finish<KP>(p, s, o, red);
 }
 
