@@ -1,7 +1,12 @@
 """Batched replicas (lceq_batch_*): every replica of a batch of uneven
 lattices with different couplings reproduces, to the bit, the chain of a
 single lattice with the same seed and couplings (same per-site code, same
-Philox counters, same reduction order), with and without hipGraph replay."""
+Philox counters, same reduction order), with and without hipGraph replay.
+
+The batch's launches always use the schedule of large cache-resident
+lattices (the operands at the site itself issued first, a fenced uniform
+phase), while these small single lattices use the other schedule, so the
+tests also pin the two schedules to the same bits."""
 import numpy as np
 import pytest
 
