@@ -831,15 +831,21 @@ extern "C" int lceq_landau_gauge(lceq_lattice* L, double* da1, int* iter_or1) {
   // remaining launches of a batch return at once.  Inside a batch the loop
   // condition for the iterations so far is tested by the next iteration's
   // first launch; one finalize tests it after the batch's last iteration.
+  // With more than 2048 blocks per launch every block's re-sum of all the
+  // partials costs more than a one-block test launch per iteration (26 us
+  // of a 352 us iteration at 160^3), so there each iteration is followed by
+  // the finalize instead: the same sum in the same order, the same stop.
+  const bool test_launch = blocks_for(L->g.n) > 2048;
   int batch = 8, done_iter = 0;
   double o[LCEQ_OBS_N];
   for (;;) {
     for (int k = 0; k < batch; ++k) {
-      launch_gfix(L, 0, 0, 0, 1, k > 0, done_iter);
+      launch_gfix(L, 0, 0, 0, 1, test_launch ? 0 : k > 0, done_iter);
       launch_gfix(L, 1, 1, 1, 1);
       ++done_iter;
+      if (test_launch) launch_gfix_finalize(L, 1, done_iter);
     }
-    launch_gfix_finalize(L, 1, done_iter);
+    if (!test_launch) launch_gfix_finalize(L, 1, done_iter);
     int rc = check_launch();
     if (rc) return rc;
     if ((rc = read_obs(L, o))) return rc;

@@ -38,8 +38,12 @@ def _host_loop(shape, u, a, lam, alpha, limit, nmax):
 
 
 @pytest.mark.parametrize("shape,alpha,limit", [((8, 8, 8), 1.0, 1e-8), ((6, 4, 8), 0.7, 1e-9),
-                                               ((16, 16, 16), 1.0, 1e-7), ((4, 4, 4), 1.0, 1e-12)])
+                                               ((16, 16, 16), 1.0, 1e-7), ((4, 4, 4), 1.0, 1e-12),
+                                               ((128, 128, 80), 1.0, 1e-4)])
 def test_device_loop_equals_host_loop(shape, alpha, limit):
+    """128 x 128 x 80 has 2560 blocks per launch: the loop condition is then
+    tested by a one-block launch after every iteration instead of inside the
+    next iteration's first launch."""
     import lceqcd_amd
     print("landau_gauge device vs host loop", shape, alpha, limit, flush=True)
     u, a = _thermalised(shape, 41)
@@ -59,10 +63,10 @@ def test_device_loop_equals_host_loop(shape, alpha, limit):
     assert G.landau_gauge() == (da, it)
 
 
-@pytest.mark.parametrize("cap", [1, 8, 13, 40])
-def test_iteration_cap(cap):
+@pytest.mark.parametrize("cap,shape", [(1, (8, 8, 8)), (8, (8, 8, 8)), (13, (8, 8, 8)),
+                                       (40, (8, 8, 8)), (13, (128, 128, 80))])
+def test_iteration_cap(cap, shape):
     import lceqcd_amd
-    shape = (8, 8, 8)
     print("landau_gauge cap", cap, flush=True)
     u, a = _thermalised(shape, 43)
     G = lceqcd_amd.Lattice(*shape)
