@@ -325,3 +325,51 @@ def test_unitarity_and_or_invariance_64(make):
     s1 = plaquette_sum((64, 64, 64), G.get_links())
     assert acc == 1.0
     assert abs(s1 - s0) <= 1e-12 * abs(s0)
+
+
+def test_large_lattice_schedule_fed_and_philox(make, omake):
+    """48 x 48 x 32 has 36,864 sites per parity, enough for the launch
+    schedule of large cache-resident lattices (the operands at the site
+    itself issued first, a fenced uniform phase); the smaller lattices above
+    run the other schedule.  With the reference's uniform stream fed and with
+    Philox streams: agauge, link overrelaxation and ahiggs1, each on
+    identical inputs, GPU == restatement within 1e-12 with no accept/reject
+    flip and equal accept counts, and the same uniforms consumed."""
+    shape = (48, 48, 32)
+    n = shape[0] * shape[1] * shape[2] // 2
+    print("large-lattice schedule", shape, flush=True)
+    G, O = _pair(make, omake, shape, 6, coup=PROD, asig=1.0)
+
+    def sync():
+        O.put_links(G.get_links())
+        O.put_higgs(G.get_higgs())
+
+    rng = np.random.default_rng(17)
+    for fed in (True, False):
+        if fed:
+            uni = rng.random((42 + 6 + 8) * n)
+            for L in (G, O):
+                L.rng_feed(uni)
+        else:
+            for L in (G, O):
+                L.rng_philox(4242, 3)
+        sync()
+        assert_close(G.agauge(1), O.agauge(1), 1e-12, "agauge accept")
+        dev, flips = _compare_sites(G.get_links(), O.get_links(), "agauge fed=%s" % fed)
+        assert flips == 0 and dev < 1e-12
+        sync()
+        assert G.link_or(1) == O.link_or(1)
+        dev, flips = _compare_sites(G.get_links(), O.get_links(), "link OR fed=%s" % fed)
+        assert flips == 0 and dev < 1e-12
+        sync()
+        dG, aG = G.ahiggs1(0.4, 1)
+        dO, aO = O.ahiggs1(0.4, 1)
+        assert aG == aO and dG == dO
+        dev, flips = _compare_sites(G.get_higgs(), O.get_higgs(), "ahiggs1 fed=%s" % fed)
+        assert flips == 0 and dev < 1e-12
+        # fed: the uniforms consumed (all of them); Philox: the pass counter
+        k = 0 if fed else 1
+        assert int(G.rng_state()[k]) == int(O.rng_state()[k])
+        if fed:
+            assert int(G.rng_state()[0]) == len(uni)
+    G.close()
